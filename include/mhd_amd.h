@@ -1,0 +1,137 @@
+/*
+ * mhd_amd.h -- C ABI of the MI355X MHD time-step library (libmhd_amd.so).
+ *
+ * One context = one box of the Kelvin-Helmholtz MHD problem resident in HBM.  The
+ * entry points mirror the functions the src_cpp_kokkos_host step loop calls
+ * (src_cpp_kokkos_host/main.cpp:229-259); each declaration names the reference
+ * function it replaces.  Plain pointers and sizes only; every function returns 0 on
+ * success, a negative MHD_ERR_* on failure and never throws.  Host arrays use the
+ * reference FieldArray layout: element (n,k,j,i) of an array with nv variables at
+ * ((k*jtot + j)*nv + n)*itot + i (src_cpp_kokkos_host/mhd.hpp:164-175), with
+ * itot = nx+5, jtot = ny+5, ktot = nz+5 (src_cpp_kokkos_host/config.hpp:56-58).
+ *
+ * Threading: one host thread per context.  Calls on one context are ordered on the
+ * context's stream; those that return host values synchronise that stream.
+ */
+#ifndef MHD_AMD_H_
+#define MHD_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD_AMD_VERSION 1
+
+enum {
+  MHD_OK = 0,
+  MHD_ERR_ARG = -1,        /* invalid argument or configuration */
+  MHD_ERR_HIP = -2,        /* a HIP runtime call failed */
+  MHD_ERR_NOMEM = -3,      /* device allocation failed */
+  MHD_ERR_UNSUPPORTED = -4,
+  MHD_ERR_STATE = -5       /* call out of order (e.g. no grid set) */
+};
+
+/* physics and grid of one box; the reference fixes these at compile time
+ * (src_cpp_kokkos_host/config.hpp:10-42, mhd.cpp:27,1560,1614) */
+typedef struct mhd_config {
+  int nx, ny, nz;          /* interior cells per axis (ngridtotal1..3, one tile) */
+  int bc[6];               /* boundary_xin,xout,yin,yout,zin,zout: 1 periodic, 2 reflection, 3 outflow */
+  double xmin[3], xmax[3]; /* x1min..x3max */
+  double gam;              /* adiabatic index; the reference C++ uses 1.4 */
+  double cfl;              /* dt = cfl*dtmin; the reference uses 0.05 */
+  double alphabp;          /* psi damping constant; the reference uses 0.1 */
+  int energy_flux;         /* 0: (E+pt)v_n - |B|^2 B_n, as src_cpp_kokkos_host/mhd.cpp:1049-1050;
+                              1: (E+pt)v_n - (v.B) B_n, as src_cpp_omp_device/mhd.cpp:523-526 */
+  int grid_shift;          /* 1: x1a(i) = dx*(i-(ngh+1))+xmin (src_cpp_kokkos_host/main.cpp:39-41);
+                              0: dx*(i-ngh)+xmin */
+  int precision;           /* 0: FP64 throughout; 1: reconstruction + HLLD in FP32 */
+  int device;              /* HIP device ordinal */
+} mhd_config;
+
+/* single-shear-layer initial state (see mhd_generate_problem_shear) */
+typedef struct mhd_ic_params {
+  double rho0, p0, v0, a, bx0, amp, sigma;
+  uint64_t seed;
+} mhd_ic_params;
+
+typedef struct mhd_ctx mhd_ctx;
+
+int mhd_version(void);
+const char* mhd_error_string(int code);
+/* last error message recorded on this thread (HIP error text or argument check) */
+const char* mhd_last_error(void);
+
+/* AllocateHydroVariables (src_cpp_kokkos_host/mhd.cpp:33-77) + AllocateBoundaryVariables
+ * (src_cpp_kokkos_host/boundary.cpp:36-94): allocates P, U in HBM, zero-filled. */
+int mhd_create(const mhd_config* cfg, mhd_ctx** out);
+/* DeallocateHydroVariables / DeallocateBoundaryVariables */
+int mhd_destroy(mhd_ctx* ctx);
+int mhd_get_dims(const mhd_ctx* ctx, int* itot, int* jtot, int* ktot);
+/* device bytes held by the context */
+int mhd_device_bytes(const mhd_ctx* ctx, uint64_t* bytes);
+
+/* GridArray upload: x1a,x1b (itot entries), x2a,x2b (jtot), x3a,x3b (ktot), as
+ * src_cpp_kokkos_host/mhd.hpp:67-110 holds them */
+int mhd_set_grid(mhd_ctx* ctx, const double* x1a, const double* x1b, const double* x2a, const double* x2b,
+                 const double* x3a, const double* x3b);
+/* GenerateGrid (src_cpp_kokkos_host/main.cpp:26-65) computed on the device */
+int mhd_generate_grid(mhd_ctx* ctx);
+int mhd_get_grid(mhd_ctx* ctx, double* x1a, double* x1b, double* x2a, double* x2b, double* x3a, double* x3b);
+
+/* FieldArray h2d/d2h (src_cpp_kokkos_device/mhd.hpp FieldArray::h2d/d2h): whole arrays,
+ * ghosts included, host layout above.  P has 12 variables, U and F have 10. */
+int mhd_upload_prim(mhd_ctx* ctx, const double* P);
+int mhd_upload_cons(mhd_ctx* ctx, const double* U);
+int mhd_download_prim(mhd_ctx* ctx, double* P);
+int mhd_download_cons(mhd_ctx* ctx, double* U);
+int mhd_download_flux(mhd_ctx* ctx, int dir, double* F);
+
+/* ---- the reference step functions, one call each ---- */
+/* ControlTimestep (src_cpp_kokkos_host/mhd.cpp:1511-1561): *dt = cfl*min CFL bound of P */
+int mhd_control_timestep(mhd_ctx* ctx, double* dt);
+/* SetBoundaryCondition (src_cpp_kokkos_host/boundary.cpp:750-893) on P */
+int mhd_set_boundary(mhd_ctx* ctx);
+/* EvaluateCh (src_cpp_kokkos_host/mhd.cpp:1563-1611): *chg = max fast speed of P */
+int mhd_evaluate_ch(mhd_ctx* ctx, double* chg);
+/* GetNumericalFluxD (src_cpp_kokkos_host/mhd.cpp:987-1410): F_dir from P, dir = 1,2,3 */
+int mhd_numerical_flux(mhd_ctx* ctx, int dir, double chg);
+/* UpdateConservU (src_cpp_kokkos_host/mhd.cpp:1412-1444) */
+int mhd_update_conserv(mhd_ctx* ctx, double dt);
+/* DampPsi (src_cpp_kokkos_host/mhd.cpp:1613-1633) */
+int mhd_damp_psi(mhd_ctx* ctx, double dt, double chg);
+/* UpdatePrimitvP (src_cpp_kokkos_host/mhd.cpp:1447-1509) */
+int mhd_update_primitive(mhd_ctx* ctx);
+
+/* ---- the whole loop body on the device ----
+ * nsteps iterations of src_cpp_kokkos_host/main.cpp:229-259 with dt and c_h kept in
+ * HBM.  *time is read as the starting time_sim and returns the final one; with
+ * tmax > 0 the loop stops after the step that takes time past tmax (main.cpp:265).
+ * dt_log (may be NULL, capacity nsteps) receives each dt.  *steps_done (may be NULL)
+ * returns the steps applied.  P and U are both valid afterwards.
+ * mode: 0 = fused step kernel (default), 1 = staged kernels (one per reference function). */
+int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done);
+
+/* Kelvin-Helmholtz initial state of src_cpp_kokkos_host/main.cpp:66-184 on the device
+ * (vy amplitude vy_amp: 1e-3 there, 1e-2 in src_cpp_omp_device; harmonic vx kick rrv). */
+int mhd_generate_problem_kh(mhd_ctx* ctx, double vy_amp, double rrv);
+/* single shear layer: rho=rho0, p=p0, vx=v0*tanh(y/a), Bx=bx0, X=(1+tanh(y/a))/2,
+ * vy = amp*exp(-(y/sigma)^2)*(2u-1), u=(splitmix64(seed ^ ((k*ny+j)*nx+i))>>11)*2^-53 */
+int mhd_generate_problem_shear(mhd_ctx* ctx, const mhd_ic_params* ic);
+
+/* t-prof diagnostics of RealTimeAnalysis (src_cpp_omp_device/main.cpp:23-69) on P:
+ * out[0]=volume, out[1]=mix, out[2]=sqrt(<vy^2>), out[3]=max|divB| (central differences
+ * on cells whose neighbours are interior; the reference has no divB diagnostic). */
+int mhd_diagnostics(mhd_ctx* ctx, double* out4);
+/* status word: bit 0 = a non-finite or non-positive dt / c_h was met */
+int mhd_status(mhd_ctx* ctx, int* status);
+
+/* kernel timing of the most recent mhd_run: total ms of the step kernels on the
+ * context's stream, measured with HIP events, and the launches they cover */
+int mhd_last_run_timing(mhd_ctx* ctx, double* kernel_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,495 @@
+// mhd_capi.hip -- the C ABI (include/mhd_amd.h): box contexts, HBM allocation, host
+// layout conversion, the reference-shaped step functions and the device-resident loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd_amd.h"
+#include "mhd_device.hpp"
+
+using mhd::Geom;
+using mhd::StepScalars;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      return fail(MHD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+#define CHKRET(expr)            \
+  do {                          \
+    int r_ = (expr);            \
+    if (r_ != MHD_OK) return r_; \
+  } while (0)
+
+}  // namespace
+
+struct mhd_ctx {
+  mhd_config cfg{};
+  Geom g{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  double* P = nullptr;
+  double* U = nullptr;
+  double* F[3] = {nullptr, nullptr, nullptr};
+  double* U2 = nullptr;  // second conserved buffer of the fused loop (ping-pong)
+  double* grid[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // x1a x1b x2a x2b x3a x3b
+  double* staging = nullptr;
+  double* diag = nullptr;
+  double* dt_log = nullptr;
+  int dt_log_cap = 0;
+  StepScalars* sc = nullptr;
+  bool grid_set = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  uint64_t bytes = 0;
+  // captured one-step graph of the device loop
+  hipGraphExec_t graph = nullptr;
+  int graph_mode = -1;
+  double* graph_log = nullptr;
+};
+
+namespace {
+
+size_t host_cells(const mhd_ctx* c) { return (size_t)c->g.itot * c->g.jtot * c->g.ktot; }
+
+int dalloc(mhd_ctx* c, double** p, size_t n) {
+  if (*p) return MHD_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(double));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(MHD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  c->bytes += n * sizeof(double);
+  HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(double), c->stream));
+  return MHD_OK;
+}
+
+int ensure_flux(mhd_ctx* c) {
+  for (int d = 0; d < 3; ++d) CHKRET(dalloc(c, &c->F[d], mhd::NCONS * c->g.vstride));
+  return MHD_OK;
+}
+
+int ensure_staging(mhd_ctx* c) { return dalloc(c, &c->staging, mhd::NPRIM * host_cells(c)); }
+
+int set_dev(mhd_ctx* c) {
+  HIPCHK(hipSetDevice(c->dev));
+  return MHD_OK;
+}
+
+void drop_graph(mhd_ctx* c) {
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_mode = -1;
+}
+
+int check_cfg(const mhd_config* cfg) {
+  if (!cfg) return fail(MHD_ERR_ARG, "null config");
+  if (cfg->nx < 4 || cfg->ny < 4 || cfg->nz < 4) return fail(MHD_ERR_ARG, "each axis needs at least 4 cells");
+  if ((long long)(cfg->nx + 5) * (cfg->ny + 5) * (cfg->nz + 5) > (1LL << 30))
+    return fail(MHD_ERR_ARG, "box too large for 32-bit cell indices");
+  for (int q = 0; q < 6; ++q)
+    if (cfg->bc[q] < 1 || cfg->bc[q] > 3) return fail(MHD_ERR_ARG, "boundary codes are 1, 2 or 3");
+  for (int a = 0; a < 3; ++a)
+    if (!(cfg->xmax[a] > cfg->xmin[a])) return fail(MHD_ERR_ARG, "empty domain");
+  if (!(cfg->gam > 1.0)) return fail(MHD_ERR_ARG, "gam must exceed 1");
+  if (!(cfg->cfl > 0.0)) return fail(MHD_ERR_ARG, "cfl must be positive");
+  if (cfg->energy_flux < 0 || cfg->energy_flux > 1) return fail(MHD_ERR_ARG, "energy_flux is 0 or 1");
+  if (cfg->precision < 0 || cfg->precision > 1) return fail(MHD_ERR_ARG, "precision is 0 or 1");
+  return MHD_OK;
+}
+
+int read_scalars(mhd_ctx* c, StepScalars* h) {
+  HIPCHK(hipMemcpyAsync(h, c->sc, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+double bits2d(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_version(void) { return MHD_AMD_VERSION; }
+
+const char* mhd_error_string(int code) {
+  switch (code) {
+    case MHD_OK: return "ok";
+    case MHD_ERR_ARG: return "invalid argument";
+    case MHD_ERR_HIP: return "HIP runtime error";
+    case MHD_ERR_NOMEM: return "device allocation failed";
+    case MHD_ERR_UNSUPPORTED: return "unsupported";
+    case MHD_ERR_STATE: return "call out of order";
+    default: return "unknown error";
+  }
+}
+
+const char* mhd_last_error(void) { return g_last_error.c_str(); }
+
+int mhd_destroy(mhd_ctx* c) {
+  if (!c) return MHD_OK;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->staging, c->diag, c->dt_log,
+                    c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5]};
+  for (double* b : bufs)
+    if (b) hipFree(b);
+  if (c->sc) hipFree(c->sc);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MHD_OK;
+}
+
+int mhd_create(const mhd_config* cfg, mhd_ctx** out) {
+  if (!out) return fail(MHD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  CHKRET(check_cfg(cfg));
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(MHD_ERR_ARG, "no such HIP device");
+  mhd_ctx* c = new (std::nothrow) mhd_ctx();
+  if (!c) return fail(MHD_ERR_NOMEM, "host allocation failed");
+  c->cfg = *cfg;
+  c->dev = cfg->device;
+  Geom& g = c->g;
+  g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
+  g.itot = cfg->nx + 5; g.jtot = cfg->ny + 5; g.ktot = cfg->nz + 5;
+  g.pitch = ((g.itot + mhd::IOFF) + 15) / 16 * 16;
+  g.vstride = (size_t)g.ktot * g.jtot * g.pitch;
+  int rc = MHD_OK;
+  auto bail = [&](int r) { mhd_destroy(c); return r; };
+  if ((rc = set_dev(c)) != MHD_OK) return bail(rc);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(MHD_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(MHD_ERR_HIP, "hipEventCreate failed"));
+  if ((rc = dalloc(c, &c->P, mhd::NPRIM * g.vstride)) != MHD_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->U, mhd::NCONS * g.vstride)) != MHD_OK) return bail(rc);
+  const int tot[3] = {g.itot, g.jtot, g.ktot};
+  for (int q = 0; q < 6; ++q)
+    if ((rc = dalloc(c, &c->grid[q], tot[q / 2])) != MHD_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->diag, 4 * (size_t)g.nz)) != MHD_OK) return bail(rc);
+  if (hipMalloc((void**)&c->sc, sizeof(StepScalars)) != hipSuccess) return bail(fail(MHD_ERR_NOMEM, "hipMalloc"));
+  if (hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream) != hipSuccess)
+    return bail(fail(MHD_ERR_HIP, "hipMemsetAsync"));
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(MHD_ERR_HIP, "hipStreamSynchronize"));
+  *out = c;
+  return MHD_OK;
+}
+
+int mhd_get_dims(const mhd_ctx* c, int* itot, int* jtot, int* ktot) {
+  if (!c || !itot || !jtot || !ktot) return fail(MHD_ERR_ARG, "null argument");
+  *itot = c->g.itot; *jtot = c->g.jtot; *ktot = c->g.ktot;
+  return MHD_OK;
+}
+
+int mhd_device_bytes(const mhd_ctx* c, uint64_t* bytes) {
+  if (!c || !bytes) return fail(MHD_ERR_ARG, "null argument");
+  *bytes = c->bytes + sizeof(StepScalars);
+  return MHD_OK;
+}
+
+int mhd_set_grid(mhd_ctx* c, const double* x1a, const double* x1b, const double* x2a, const double* x2b,
+                 const double* x3a, const double* x3b) {
+  if (!c || !x1a || !x1b || !x2a || !x2b || !x3a || !x3b) return fail(MHD_ERR_ARG, "null argument");
+  CHKRET(set_dev(c));
+  const double* src[6] = {x1a, x1b, x2a, x2b, x3a, x3b};
+  const int tot[3] = {c->g.itot, c->g.jtot, c->g.ktot};
+  for (int q = 0; q < 6; ++q)
+    HIPCHK(hipMemcpyAsync(c->grid[q], src[q], tot[q / 2] * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->grid_set = true;
+  return MHD_OK;
+}
+
+int mhd_generate_grid(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  CHKRET(set_dev(c));
+  mhd::launch_generate_grid(c->g, c->cfg.xmin, c->cfg.xmax, c->cfg.grid_shift, c->grid[0], c->grid[1], c->grid[2],
+                            c->grid[3], c->grid[4], c->grid[5], c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->grid_set = true;
+  return MHD_OK;
+}
+
+int mhd_get_grid(mhd_ctx* c, double* x1a, double* x1b, double* x2a, double* x2b, double* x3a, double* x3b) {
+  if (!c || !x1a || !x1b || !x2a || !x2b || !x3a || !x3b) return fail(MHD_ERR_ARG, "null argument");
+  CHKRET(set_dev(c));
+  double* dst[6] = {x1a, x1b, x2a, x2b, x3a, x3b};
+  const int tot[3] = {c->g.itot, c->g.jtot, c->g.ktot};
+  for (int q = 0; q < 6; ++q)
+    HIPCHK(hipMemcpyAsync(dst[q], c->grid[q], tot[q / 2] * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+static int upload(mhd_ctx* c, int nv, const double* host, double* dev) {
+  if (!c || !host) return fail(MHD_ERR_ARG, "null argument");
+  CHKRET(set_dev(c));
+  CHKRET(ensure_staging(c));
+  HIPCHK(hipMemcpyAsync(c->staging, host, nv * host_cells(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  mhd::launch_from_host_layout(c->g, nv, c->staging, dev, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+static int download(mhd_ctx* c, int nv, const double* dev, double* host) {
+  if (!c || !host) return fail(MHD_ERR_ARG, "null argument");
+  if (!dev) return fail(MHD_ERR_STATE, "array not computed yet");
+  CHKRET(set_dev(c));
+  CHKRET(ensure_staging(c));
+  mhd::launch_to_host_layout(c->g, nv, dev, c->staging, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, c->staging, nv * host_cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_upload_prim(mhd_ctx* c, const double* P) { return upload(c, mhd::NPRIM, P, c ? c->P : nullptr); }
+int mhd_upload_cons(mhd_ctx* c, const double* U) { return upload(c, mhd::NCONS, U, c ? c->U : nullptr); }
+int mhd_download_prim(mhd_ctx* c, double* P) { return download(c, mhd::NPRIM, c ? c->P : nullptr, P); }
+int mhd_download_cons(mhd_ctx* c, double* U) { return download(c, mhd::NCONS, c ? c->U : nullptr, U); }
+int mhd_download_flux(mhd_ctx* c, int dir, double* F) {
+  if (!c || dir < 1 || dir > 3) return fail(MHD_ERR_ARG, "dir is 1, 2 or 3");
+  return download(c, mhd::NCONS, c->F[dir - 1], F);
+}
+
+static int need_grid(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  if (!c->grid_set) return fail(MHD_ERR_STATE, "grid not set (mhd_set_grid or mhd_generate_grid first)");
+  return set_dev(c);
+}
+
+static int host_reduce(mhd_ctx* c, double* dtmin, double* ch) {
+  mhd::launch_reset_scalars(c->sc, 0, c->stream);
+  mhd::launch_reduce_prim(c->g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 0, c->stream);
+  HIPCHK(hipGetLastError());
+  StepScalars h;
+  CHKRET(read_scalars(c, &h));
+  if (dtmin) *dtmin = bits2d(h.host_acc[0]);
+  if (ch) *ch = bits2d(h.host_acc[1]);
+  return MHD_OK;
+}
+
+int mhd_control_timestep(mhd_ctx* c, double* dt) {
+  CHKRET(need_grid(c));
+  if (!dt) return fail(MHD_ERR_ARG, "null dt");
+  double dtmin;
+  CHKRET(host_reduce(c, &dtmin, nullptr));
+  *dt = c->cfg.cfl * dtmin;
+  return MHD_OK;
+}
+
+int mhd_evaluate_ch(mhd_ctx* c, double* chg) {
+  CHKRET(need_grid(c));
+  if (!chg) return fail(MHD_ERR_ARG, "null chg");
+  return host_reduce(c, nullptr, chg);
+}
+
+int mhd_set_boundary(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  CHKRET(set_dev(c));
+  mhd::launch_boundary_prim(c->g, c->cfg.bc, c->P, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_numerical_flux(mhd_ctx* c, int dir, double chg) {
+  if (!c || dir < 1 || dir > 3) return fail(MHD_ERR_ARG, "dir is 1, 2 or 3");
+  CHKRET(set_dev(c));
+  CHKRET(ensure_flux(c));
+  mhd::launch_flux(c->g, dir, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[dir - 1], chg, nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_update_conserv(mhd_ctx* c, double dt) {
+  CHKRET(need_grid(c));
+  if (!c->F[0] || !c->F[1] || !c->F[2]) return fail(MHD_ERR_STATE, "fluxes not computed");
+  mhd::launch_update(c->g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, dt, c->cfg.cfl,
+                     c->cfg.alphabp, nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_damp_psi(mhd_ctx* c, double dt, double chg) {
+  CHKRET(need_grid(c));
+  mhd::launch_damp(c->g, c->grid[0], c->grid[2], c->grid[4], c->U, dt, chg, c->cfg.alphabp, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_update_primitive(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  CHKRET(set_dev(c));
+  mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+// launches of one loop iteration; every argument is iteration-invariant
+static int issue_step(mhd_ctx* c, int mode, int* launches) {
+  const Geom& g = c->g;
+  (void)mode;
+  {
+    mhd::launch_reduce_prim(g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
+    mhd::launch_boundary_prim(g, c->cfg.bc, c->P, c->stream);
+    for (int d = 1; d <= 3; ++d)
+      mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
+    mhd::launch_update(g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, 0.0, c->cfg.cfl,
+                       c->cfg.alphabp, c->sc, c->stream);
+    mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
+    mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, c->stream);
+    *launches = 8;
+  }
+  HIPCHK(hipGetLastError());
+  return MHD_OK;
+}
+
+int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done) {
+  CHKRET(need_grid(c));
+  if (nsteps < 0 || !time) return fail(MHD_ERR_ARG, "nsteps >= 0 and a time pointer are required");
+  if (mode != 0 && mode != 1) return fail(MHD_ERR_ARG, "mode is 0 (fused) or 1 (staged)");
+  // device dt log, grown as needed (the captured graph holds its address)
+  if (nsteps > c->dt_log_cap) {
+    drop_graph(c);
+    if (c->dt_log) { (void)hipFree(c->dt_log); c->bytes -= c->dt_log_cap * sizeof(double); c->dt_log = nullptr; }
+    CHKRET(dalloc(c, &c->dt_log, std::max(nsteps, 1)));
+    c->dt_log_cap = std::max(nsteps, 1);
+  }
+  CHKRET(ensure_flux(c));
+  // loop state
+  StepScalars h{};
+  h.time = *time;
+  h.tmax = tmax;
+  h.dt_log_cap = c->dt_log_cap;
+  HIPCHK(hipMemcpyAsync(&c->sc->time, &h.time, sizeof(double) * 2, hipMemcpyHostToDevice, c->stream));
+  {
+    int zeros[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(&c->sc->iter, zeros, sizeof(int) * 3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, &h.dt_log_cap, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  mhd::launch_reset_scalars(c->sc, 1, c->stream);
+  HIPCHK(hipGetLastError());
+  // capture one iteration once, then replay it
+  if (!c->graph || c->graph_mode != mode || c->graph_log != c->dt_log) {
+    drop_graph(c);
+    hipStream_t cap = c->stream;
+    HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int launches = 0;
+    int rc = issue_step(c, mode, &launches);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (rc != MHD_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(MHD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { c->graph = nullptr; return fail(MHD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    c->graph_mode = mode;
+    c->graph_log = c->dt_log;
+    c->last_launches = launches;
+  }
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  CHKRET(read_scalars(c, &h));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->last_launches = c->last_launches * nsteps / std::max(nsteps, 1) * (nsteps > 0 ? 1 : 0);
+  if (dt_log && h.steps_done > 0)
+    HIPCHK(hipMemcpy(dt_log, c->dt_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
+  *time = h.time;
+  if (steps_done) *steps_done = h.steps_done;
+  return MHD_OK;
+}
+
+int mhd_generate_problem_kh(mhd_ctx* c, double vy_amp, double rrv) {
+  CHKRET(need_grid(c));
+  mhd::launch_problem_kh(c->g, c->cfg.gam, vy_amp, rrv, c->cfg.xmin, c->cfg.xmax, c->grid[1], c->grid[3], c->grid[5],
+                         c->P, c->U, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_generate_problem_shear(mhd_ctx* c, const mhd_ic_params* ic) {
+  CHKRET(need_grid(c));
+  if (!ic || !(ic->a > 0.0) || !(ic->rho0 > 0.0) || !(ic->p0 > 0.0) || !(ic->sigma > 0.0))
+    return fail(MHD_ERR_ARG, "shear parameters need a, rho0, p0, sigma > 0");
+  const double v[7] = {ic->rho0, ic->p0, ic->v0, ic->a, ic->bx0, ic->amp, ic->sigma};
+  mhd::launch_problem_shear(c->g, c->cfg.gam, v, ic->seed, c->grid[3], c->P, c->U, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_diagnostics(mhd_ctx* c, double* out4) {
+  CHKRET(need_grid(c));
+  if (!out4) return fail(MHD_ERR_ARG, "null output");
+  mhd::launch_diagnostics(c->g, c->P, c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5], c->diag,
+                          c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part(4 * (size_t)c->g.nz);
+  HIPCHK(hipMemcpyAsync(part.data(), c->diag, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double sv = 0.0, sm = 0.0, sy = 0.0, md = 0.0;
+  for (int k = 0; k < c->g.nz; ++k) {
+    sv += part[4 * k]; sm += part[4 * k + 1]; sy += part[4 * k + 2]; md = std::max(md, part[4 * k + 3]);
+  }
+  out4[0] = sv;
+  out4[1] = sm / sv;
+  out4[2] = std::sqrt(sy / sv);
+  out4[3] = md;
+  return MHD_OK;
+}
+
+int mhd_status(mhd_ctx* c, int* status) {
+  if (!c || !status) return fail(MHD_ERR_ARG, "null argument");
+  CHKRET(set_dev(c));
+  StepScalars h;
+  CHKRET(read_scalars(c, &h));
+  *status = h.status;
+  return MHD_OK;
+}
+
+int mhd_last_run_timing(mhd_ctx* c, double* kernel_ms, int* launches) {
+  if (!c || !kernel_ms || !launches) return fail(MHD_ERR_ARG, "null argument");
+  *kernel_ms = c->last_ms;
+  *launches = c->last_launches;
+  return MHD_OK;
+}
+
+}  // extern "C"

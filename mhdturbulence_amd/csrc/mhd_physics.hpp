@@ -1,0 +1,316 @@
+// mhd_physics.hpp -- per-point arithmetic of the MHD step for CDNA4 (gfx950).
+//
+// Device functions shared by the staged kernels (one launch per reference function)
+// and the fused step kernel.  They are templated on the reconstruction / Riemann
+// precision R (double, or float for the mixed-precision path) and on the sweep
+// direction D (0 = x, 1 = y, 2 = z) so each direction compiles to straight-line code
+// with the component permutation resolved at compile time.
+//
+// Reference: src_cpp_kokkos_host/mhd.cpp (cited as KH/mhd.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mhd_device.hpp"
+
+namespace mhd {
+
+enum Prim { nden = 0, nve1, nve2, nve3, nene, npre, ncsp, nbm1, nbm2, nbm3, nbps, nst };
+enum Cons { mden = 0, mrv1, mrv2, mrv3, meto, mbm1, mbm2, mbm3, mbps, mst };
+
+// slot of Cartesian component c in the rotated (u normal, v, w) frame of direction D:
+// D=0: (x,y,z)->(u,v,w), D=1: (y,z,x)->(u,v,w), D=2: (z,x,y)->(u,v,w)   (KH/mhd.cpp:1078-1118)
+__host__ __device__ constexpr int slot(int c, int d) { return (c - d + 3) % 3; }
+// Cartesian component sitting in rotated slot s for direction D
+__host__ __device__ constexpr int comp(int s, int d) { return (s + d) % 3; }
+
+// van Leer harmonic limiter, KH/mhd.cpp:89-95
+template <typename R>
+__device__ __forceinline__ R van_leer(R dp, R dm) {
+  const R prod = dp * dm;
+  const R v = R(2) * dp * dm / (dp + dm);
+  return prod > R(0) ? v : R(0);
+}
+
+// One interface state in the rotated frame: the quantities HLLD reads from the
+// 23-entry Riemann vector of KH/mhd.cpp:997-1076, kept as named registers.
+template <typename R>
+struct RState {
+  R rho, mu, mv, mw, E, bu, bv, bw, psi, sc;    // rotated conserved
+  R fd, fu, fv, fw, fE, fbv, fbw, fsc;           // rotated physical flux (fbu = fpsi = 0)
+  R cf, vn, pt;                                  // fast speed, normal velocity, total pressure
+};
+
+// Prim2ConsD for direction D (KH/mhd.cpp:997-1076).  EFLUX=0: energy flux magnetic term
+// |B|^2 B_n as written at KH/mhd.cpp:1049-1050; EFLUX=1: (v.B) B_n.
+template <int D, int EFLUX, typename R>
+__device__ __forceinline__ void prim_to_state(const R* q, RState<R>& s) {
+  constexpr int DF = (D + 1) % 3, DB = (D + 2) % 3;
+  const R rho = q[nden];
+  const R rhoinv = R(1) / rho;
+  const R vsq = q[nve1] * q[nve1] + q[nve2] * q[nve2] + q[nve3] * q[nve3];
+  const R bsq = q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3];
+  const R vn = q[nve1 + D], bn = q[nbm1 + D];
+  const R vf = q[nve1 + DF], bf = q[nbm1 + DF];
+  const R vb = q[nve1 + DB], bb = q[nbm1 + DB];
+  s.rho = rho;
+  s.mu = rho * vn;
+  s.mv = rho * vf;
+  s.mw = rho * vb;
+  s.E = rho * q[nene] + R(0.5) * rho * vsq + R(0.5) * bsq;
+  s.bu = bn;
+  s.bv = bf;
+  s.bw = bb;
+  s.psi = q[nbps];
+  s.sc = rho * q[nst];
+  const R pt = q[npre] + bsq / R(2);
+  s.fd = rho * vn;
+  s.fu = rho * vn * vn - bn * bn + pt;
+  s.fv = rho * vf * vn - bf * bn;
+  s.fw = rho * vb * vn - bb * bn;
+  if constexpr (EFLUX == 0) {
+    s.fE = (s.E + pt) * vn - bsq * bn;
+  } else {
+    const R vdb = q[nve1] * q[nbm1] + q[nve2] * q[nbm2] + q[nve3] * q[nbm3];
+    s.fE = (s.E + pt) * vn - vdb * bn;
+  }
+  s.fbv = bf * vn - vf * bn;
+  s.fbw = bb * vn - vb * bn;
+  s.fsc = rho * q[nst] * vn;
+  const R css = q[ncsp] * q[ncsp];
+  const R cts = css + bsq * rhoinv;
+  s.cf = sqrt((cts + sqrt(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  s.vn = vn;
+  s.pt = pt;
+}
+
+// Rotated-frame numerical flux from HLLD (Miyoshi & Kusano 2005, J. Comput. Phys. 208:315)
+// in the branch-free form of KH/mhd.cpp:97-417; the normal-B and psi slots are left to GLM.
+template <typename R>
+struct RFlux {
+  R d, u, v, w, E, bv, bw, sc;
+};
+
+// std::copysign(1, x) as the reference writes it (so -0.0 selects -1)
+template <typename R>
+__device__ __forceinline__ R copysign1(R x) {
+  return copysign(R(1), x);
+}
+
+template <typename R>
+__device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RFlux<R>& nf) {
+  const R eps = R(1.0e-30);
+  const R tiny = R(1.0e-20);
+  const R rol = L.rho, eel = L.E, rxl = L.mu, ryl = L.mv, rzl = L.mw;
+  const R vxl = L.mu / L.rho, vyl = L.mv / L.rho, vzl = L.mw / L.rho;
+  const R byl = L.bv, bzl = L.bw, ptl = L.pt, scl = L.sc;
+  const R ror = Rt.rho, eer = Rt.E, rxr = Rt.mu, ryr = Rt.mv, rzr = Rt.mw;
+  const R vxr = Rt.mu / Rt.rho, vyr = Rt.mv / Rt.rho, vzr = Rt.mw / Rt.rho;
+  const R byr = Rt.bv, bzr = Rt.bw, ptr = Rt.pt, scr = Rt.sc;
+
+  const R cfmax = fmax(L.cf, Rt.cf);
+  const R sl = fmin(vxl, vxr) - cfmax;
+  const R sr = fmax(vxl, vxr) + cfmax;
+
+  const R sdl = fmin(-tiny, sl - vxl);
+  const R sdr = fmax(tiny, sr - vxr);
+  const R rosdl = rol * sdl;
+  const R rosdr = ror * sdr;
+  R temp = R(1) / (rosdr - rosdl);
+  const R sm = (rosdr * vxr - rosdl * vxl - ptr + ptl) * temp;
+  const R sdml = fmin(-tiny, sl - sm);
+  const R sdmr = fmax(tiny, sr - sm);
+  const R ptst = (rosdr * ptl - rosdl * ptr + rosdl * rosdr * (vxr - vxl)) * temp;
+
+  // left star state (KH/mhd.cpp:268-302)
+  const R bxs = R(0.5) * (L.bu + Rt.bu);
+  const R bxsq = bxs * bxs;
+  R temp_fst = rosdl * sdml - bxsq;
+  R sign1 = copysign1(fabs(temp_fst) - eps);
+  R maxs1 = fmax(R(0), sign1);
+  R mins1 = fmin(R(0), sign1);
+  R itf = R(1) / (temp_fst + mins1);
+  const R isdml = R(1) / sdml;
+  temp = bxs * (sdl - sdml) * itf;
+  const R rolst = maxs1 * (rosdl * isdml) - mins1 * rol;
+  const R sclst = rolst / rol * scl;
+  const R vxlst = maxs1 * sm - mins1 * vxl;
+  const R rxlst = rolst * vxlst;
+  const R vylst = maxs1 * (vyl - byl * temp) - mins1 * vyl;
+  const R rylst = rolst * vylst;
+  const R vzlst = maxs1 * (vzl - bzl * temp) - mins1 * vzl;
+  const R rzlst = rolst * vzlst;
+  temp = (rosdl * sdl - bxsq) * itf;
+  const R bylst = maxs1 * (byl * temp) - mins1 * byl;
+  const R bzlst = maxs1 * (bzl * temp) - mins1 * bzl;
+  const R vdbstl = vxlst * bxs + vylst * bylst + vzlst * bzlst;
+  const R eelst =
+      maxs1 * (sdl * eel - ptl * vxl + ptst * sm + bxs * (vxl * bxs + vyl * byl + vzl * bzl - vdbstl)) * isdml -
+      mins1 * eel;
+
+  // right star state (KH/mhd.cpp:304-333)
+  temp_fst = rosdr * sdmr - bxsq;
+  sign1 = copysign1(fabs(temp_fst) - eps);
+  maxs1 = fmax(R(0), sign1);
+  mins1 = fmin(R(0), sign1);
+  itf = R(1) / (temp_fst + mins1);
+  const R isdmr = R(1) / sdmr;
+  temp = bxs * (sdr - sdmr) * itf;
+  const R rorst = maxs1 * (rosdr * isdmr) - mins1 * ror;
+  const R scrst = rorst / ror * scr;
+  const R vxrst = maxs1 * sm - mins1 * vxr;
+  const R rxrst = rorst * vxrst;
+  const R vyrst = maxs1 * (vyr - byr * temp) - mins1 * vyr;
+  const R ryrst = rorst * vyrst;
+  const R vzrst = maxs1 * (vzr - bzr * temp) - mins1 * vzr;
+  const R rzrst = rorst * vzrst;
+  temp = (rosdr * sdr - bxsq) * itf;
+  const R byrst = maxs1 * (byr * temp) - mins1 * byr;
+  const R bzrst = maxs1 * (bzr * temp) - mins1 * bzr;
+  const R vdbstr = vxrst * bxs + vyrst * byrst + vzrst * bzrst;
+  const R eerst = maxs1 * ((sdr * eer - ptr * vxr + ptst * sm) * isdmr +
+                           bxs * (vxr * bxs + vyr * byr + vzr * bzr - vdbstr) * isdmr) -
+                  mins1 * eer;
+
+  // double-star states (KH/mhd.cpp:337-377)
+  const R sqrtrol = sqrt(rolst);
+  const R sqrtror = sqrt(rorst);
+  const R abbx = fabs(bxs);
+  const R signbx = copysign1(bxs);
+  sign1 = copysign1(abbx - eps);
+  maxs1 = fmax(R(0), sign1);
+  mins1 = -fmin(R(0), sign1);
+  const R invsumro = maxs1 / (sqrtrol + sqrtror);
+
+  temp = invsumro * (sqrtrol * vylst + sqrtror * vyrst + signbx * (byrst - bylst));
+  const R vyldst = vylst * mins1 + temp;
+  const R ryldst = rylst * mins1 + rolst * temp;
+  const R ryrdst = ryrst * mins1 + rorst * temp;
+
+  temp = invsumro * (sqrtrol * vzlst + sqrtror * vzrst + signbx * (bzrst - bzlst));
+  const R vzldst = vzlst * mins1 + temp;
+  const R rzldst = rzlst * mins1 + rolst * temp;
+  const R rzrdst = rzrst * mins1 + rorst * temp;
+
+  temp = invsumro * (sqrtrol * byrst + sqrtror * bylst + signbx * sqrtrol * sqrtror * (vyrst - vylst));
+  const R byldst = bylst * mins1 + temp;
+  const R byrdst = byrst * mins1 + temp;
+
+  temp = invsumro * (sqrtrol * bzrst + sqrtror * bzlst + signbx * sqrtrol * sqrtror * (vzrst - vzlst));
+  const R bzldst = bzlst * mins1 + temp;
+  const R bzrdst = bzrst * mins1 + temp;
+
+  temp = sm * bxs + vyldst * byldst + vzldst * bzldst;
+  const R eeldst = eelst - sqrtrol * signbx * (vdbstl - temp) * maxs1;
+  const R eerdst = eerst + sqrtror * signbx * (vdbstr - temp) * maxs1;
+
+  // flux selection by sign(S_M) (KH/mhd.cpp:381-415)
+  const R slst = (sm - abbx / sqrtrol) * maxs1;
+  const R srst = (sm + abbx / sqrtror) * maxs1;
+  sign1 = copysign1(sm);
+  maxs1 = fmax(R(0), sign1);
+  mins1 = -fmin(R(0), sign1);
+  const R msl = fmin(sl, R(0));
+  const R msr = fmax(sr, R(0));
+  const R mslst = fmin(slst, R(0));
+  const R msrst = fmax(srst, R(0));
+
+  nf.d = (L.fd + (rolst - rol) * msl) * maxs1 + (Rt.fd + (rorst - ror) * msr) * mins1;
+  nf.E = (L.fE + (eelst - eel) * msl + (eeldst - eelst) * mslst) * maxs1 +
+         (Rt.fE + (eerst - eer) * msr + (eerdst - eerst) * msrst) * mins1;
+  nf.u = (L.fu + (rxlst - rxl) * msl) * maxs1 + (Rt.fu + (rxrst - rxr) * msr) * mins1;
+  nf.v = (L.fv + (rylst - ryl) * msl + (ryldst - rylst) * mslst) * maxs1 +
+         (Rt.fv + (ryrst - ryr) * msr + (ryrdst - ryrst) * msrst) * mins1;
+  nf.w = (L.fw + (rzlst - rzl) * msl + (rzldst - rzlst) * mslst) * maxs1 +
+         (Rt.fw + (rzrst - rzr) * msr + (rzrdst - rzrst) * msrst) * mins1;
+  nf.bv = (L.fbv + (bylst - byl) * msl + (byldst - bylst) * mslst) * maxs1 +
+          (Rt.fbv + (byrst - byr) * msr + (byrdst - byrst) * msrst) * mins1;
+  nf.bw = (L.fbw + (bzlst - bzl) * msl + (bzldst - bzlst) * mslst) * maxs1 +
+          (Rt.fbw + (bzrst - bzr) * msr + (bzrdst - bzrst) * msrst) * mins1;
+  nf.sc = (L.fsc + (sclst - scl) * msl) * maxs1 + (Rt.fsc + (scrst - scr) * msr) * mins1;
+}
+
+// Interface flux in Cartesian conserved order (F[0..9]) from reconstructed left/right
+// primitive states: HLLD for the MHD part plus the GLM (B_n, psi) flux
+// F(B_n) = (psiL+psiR)/2 - c_h (B_nR-B_nL)/2,  F(psi) = c_h^2 [(B_nL+B_nR)/2 - (psiR-psiL)/(2 c_h)]
+// (scatter and GLM as at KH/mhd.cpp:1225-1240, 1306-1321, 1391-1406).
+template <int D, int EFLUX, typename R>
+__device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double chg, double* F) {
+  RState<R> sl, sr;
+  prim_to_state<D, EFLUX, R>(ql, sl);
+  prim_to_state<D, EFLUX, R>(qr, sr);
+  RFlux<R> nf;
+  hlld<R>(sl, sr, nf);
+  F[mden] = double(nf.d);
+  F[meto] = double(nf.E);
+  F[mrv1 + D] = double(nf.u);
+  F[mrv1 + (D + 1) % 3] = double(nf.v);
+  F[mrv1 + (D + 2) % 3] = double(nf.w);
+  F[mbm1 + (D + 1) % 3] = double(nf.bv);
+  F[mbm1 + (D + 2) % 3] = double(nf.bw);
+  F[mst] = double(nf.sc);
+  // GLM part in FP64 from the reconstructed (B_n, psi)
+  const double bnl = double(sl.bu), bnr = double(sr.bu), psl = double(sl.psi), psr = double(sr.psi);
+  F[mbm1 + D] = 0.5 * (psl + psr) - 0.5 * chg * (bnr - bnl);
+  F[mbps] = (0.5 * (bnl + bnr) - 0.5 / chg * (psr - psl)) * chg * chg;
+}
+
+// Left state at face f+1/2 and right state at face f-1/2 of cell f, from cells f-1, f, f+1:
+//   PL(f+1/2) = P[f] + vL(P[f+1]-P[f], P[f]-P[f-1])/2,  PR(f-1/2) = P[f] - (same slope)/2
+// (KH/mhd.cpp:1134-1154: the same limited slope of cell f serves both faces).
+template <typename R>
+__device__ __forceinline__ void cell_states(const R* qm, const R* q0, const R* qp, R* to_right, R* to_left) {
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) {
+    const R dsv = van_leer(qp[n] - q0[n], q0[n] - qm[n]);
+    to_right[n] = q0[n] + R(0.5) * dsv;
+    to_left[n] = q0[n] - R(0.5) * dsv;
+  }
+}
+
+// UpdatePrimitvP on one cell, KH/mhd.cpp:1480-1507
+__device__ __forceinline__ void cons_to_prim(const double* u, double gam, double* q) {
+  const double rho = u[mden];
+  const double rhoinv = 1.0 / rho;
+  q[nden] = rho;
+  q[nve1] = u[mrv1] * rhoinv;
+  q[nve2] = u[mrv2] * rhoinv;
+  q[nve3] = u[mrv3] * rhoinv;
+  const double ekin = 0.5 * (u[mrv1] * u[mrv1] + u[mrv2] * u[mrv2] + u[mrv3] * u[mrv3]) * rhoinv;
+  const double emag = 0.5 * (u[mbm1] * u[mbm1] + u[mbm2] * u[mbm2] + u[mbm3] * u[mbm3]);
+  const double e = (u[meto] - ekin - emag) / rho;
+  q[nene] = e;
+  q[npre] = e * rho * (gam - 1.0);
+  q[ncsp] = sqrt(e * gam * (gam - 1.0));
+  q[nbm1] = u[mbm1];
+  q[nbm2] = u[mbm2];
+  q[nbm3] = u[mbm3];
+  q[nbps] = u[mbps];
+  q[nst] = u[mst] * rhoinv;
+}
+
+// CFL time-step bound of one cell (KH/mhd.cpp:1542-1553) with the cell widths given
+__device__ __forceinline__ double cell_dt(const double* q, double dx, double dy, double dz) {
+  const double ctot =
+      sqrt(q[ncsp] * q[ncsp] + (q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3]) / q[nden]);
+  const double d1 = dx / (fabs(q[nve1]) + ctot);
+  const double d2 = dy / (fabs(q[nve2]) + ctot);
+  const double d3 = dz / (fabs(q[nve3]) + ctot);
+  return fmin(d1, fmin(d2, d3));
+}
+
+// GLM cleaning speed bound of one cell: max_d |v_d| + c_f,d  (KH/mhd.cpp:1589-1601)
+__device__ __forceinline__ double cell_ch(const double* q) {
+  const double rho = q[nden];
+  const double css = q[ncsp] * q[ncsp];
+  const double ca1 = q[nbm1] * q[nbm1] / rho;
+  const double ca2 = q[nbm2] * q[nbm2] / rho;
+  const double ca3 = q[nbm3] * q[nbm3] / rho;
+  const double cts = css + ca1 + ca2 + ca3;
+  const double cm1 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca1)) / 2.0);
+  const double cm2 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca2)) / 2.0);
+  const double cm3 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca3)) / 2.0);
+  return fmax(fabs(q[nve1]) + cm1, fmax(fabs(q[nve2]) + cm2, fabs(q[nve3]) + cm3));
+}
+
+}  // namespace mhd

@@ -1,0 +1,206 @@
+"""Python mirror of the src_cpp_kokkos_host step interface over libmhd_amd.so.
+
+`Box` is one MHD box resident in HBM.  Its methods carry the reference function names
+(src_cpp_kokkos_host/mhd.hpp:277-287, boundary.hpp:139-144) and argument meanings;
+host arrays use the reference FieldArray layout, numpy shape (ktot, jtot, nv, itot),
+element (n,k,j,i) at [k, j, n, i].
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+NPRIM, NCONS = 12, 10
+PERIODIC, REFLECTION, OUTFLOW = 1, 2, 3
+
+# primitive / conserved variable indices (src_cpp_kokkos_host/mhd.hpp:251-268)
+nden, nve1, nve2, nve3, nene, npre, ncsp, nbm1, nbm2, nbm3, nbps, nst = range(12)
+mden, mrv1, mrv2, mrv3, meto, mbm1, mbm2, mbm3, mbps, mst = range(10)
+
+
+@dataclass
+class Config:
+    """Runtime form of src_cpp_kokkos_host/config.hpp plus the constants of mhd.cpp."""
+    nx: int = 64
+    ny: int = 64
+    nz: int = 64
+    bc: tuple = (PERIODIC, PERIODIC, REFLECTION, REFLECTION, PERIODIC, PERIODIC)
+    xmin: tuple = (-0.5, -1.0, -0.5)
+    xmax: tuple = (0.5, 1.0, 0.5)
+    gam: float = 1.4
+    cfl: float = 0.05
+    alphabp: float = 0.1
+    energy_flux: int = 0     # 0: |B|^2 B_n (src_cpp_kokkos_host); 1: (v.B) B_n
+    grid_shift: int = 1      # 1: C++ variants' face formula; 0: unshifted
+    precision: int = 0       # 0: FP64; 1: FP32 reconstruction + HLLD
+    device: int = 0
+
+    def to_c(self) -> capi.MhdConfig:
+        c = capi.MhdConfig()
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        for i, v in enumerate(self.bc):
+            c.bc[i] = v
+        for i in range(3):
+            c.xmin[i], c.xmax[i] = self.xmin[i], self.xmax[i]
+        c.gam, c.cfl, c.alphabp = self.gam, self.cfl, self.alphabp
+        c.energy_flux, c.grid_shift, c.precision, c.device = (self.energy_flux, self.grid_shift,
+                                                              self.precision, self.device)
+        return c
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Box:
+    """One box of the Kelvin-Helmholtz MHD problem in HBM (AllocateHydroVariables)."""
+
+    def __init__(self, cfg: Config | None = None, **kw):
+        self.cfg = cfg if cfg is not None else Config(**kw)
+        self.L = capi.load()
+        self._c = self.cfg.to_c()
+        h = C.c_void_p()
+        capi.check(self.L.mhd_create(C.byref(self._c), C.byref(h)))
+        self.h = h
+        it, jt, kt = C.c_int(), C.c_int(), C.c_int()
+        capi.check(self.L.mhd_get_dims(self.h, C.byref(it), C.byref(jt), C.byref(kt)))
+        self.itot, self.jtot, self.ktot = it.value, jt.value, kt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- data movement
+    def _shape(self, nv):
+        return (self.ktot, self.jtot, nv, self.itot)
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        capi.check(self.L.mhd_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def set_grid(self, x1a, x1b, x2a, x2b, x3a, x3b):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x1a, x1b, x2a, x2b, x3a, x3b)]
+        capi.check(self.L.mhd_set_grid(self.h, *[_dp(a) for a in arrs]))
+
+    def generate_grid(self):
+        capi.check(self.L.mhd_generate_grid(self.h))
+
+    def get_grid(self):
+        arrs = [np.zeros(n) for n in (self.itot, self.itot, self.jtot, self.jtot, self.ktot, self.ktot)]
+        capi.check(self.L.mhd_get_grid(self.h, *[_dp(a) for a in arrs]))
+        return arrs
+
+    def upload_prim(self, P: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == self._shape(NPRIM), P.shape
+        capi.check(self.L.mhd_upload_prim(self.h, _dp(P)))
+
+    def upload_cons(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == self._shape(NCONS), U.shape
+        capi.check(self.L.mhd_upload_cons(self.h, _dp(U)))
+
+    def download_prim(self) -> np.ndarray:
+        P = np.zeros(self._shape(NPRIM))
+        capi.check(self.L.mhd_download_prim(self.h, _dp(P)))
+        return P
+
+    def download_cons(self) -> np.ndarray:
+        U = np.zeros(self._shape(NCONS))
+        capi.check(self.L.mhd_download_cons(self.h, _dp(U)))
+        return U
+
+    def download_flux(self, d: int) -> np.ndarray:
+        F = np.zeros(self._shape(NCONS))
+        capi.check(self.L.mhd_download_flux(self.h, d, _dp(F)))
+        return F
+
+    # ---- the reference step functions
+    def control_timestep(self) -> float:          # ControlTimestep
+        dt = C.c_double()
+        capi.check(self.L.mhd_control_timestep(self.h, C.byref(dt)))
+        return dt.value
+
+    def set_boundary_condition(self):              # SetBoundaryCondition
+        capi.check(self.L.mhd_set_boundary(self.h))
+
+    def evaluate_ch(self) -> float:                # EvaluateCh
+        ch = C.c_double()
+        capi.check(self.L.mhd_evaluate_ch(self.h, C.byref(ch)))
+        return ch.value
+
+    def get_numerical_flux(self, d: int, chg: float):  # GetNumericalFluxD
+        capi.check(self.L.mhd_numerical_flux(self.h, d, chg))
+
+    def update_conserv_u(self, dt: float):         # UpdateConservU
+        capi.check(self.L.mhd_update_conserv(self.h, dt))
+
+    def damp_psi(self, dt: float, chg: float):     # DampPsi
+        capi.check(self.L.mhd_damp_psi(self.h, dt, chg))
+
+    def update_primitiv_p(self):                   # UpdatePrimitvP
+        capi.check(self.L.mhd_update_primitive(self.h))
+
+    def step_reference_order(self, time: float) -> tuple[float, float]:
+        """One loop body of src_cpp_kokkos_host/main.cpp:229-259, one call per function."""
+        dt = self.control_timestep()
+        self.set_boundary_condition()
+        chg = self.evaluate_ch()
+        for d in (1, 2, 3):
+            self.get_numerical_flux(d, chg)
+        self.update_conserv_u(dt)
+        self.damp_psi(dt, chg)
+        self.update_primitiv_p()
+        return time + dt, dt
+
+    def run(self, nsteps: int, time: float = 0.0, tmax: float = 0.0, mode: int = 0):
+        """nsteps loop bodies on the device; returns (time, dt_log, steps_done)."""
+        t = C.c_double(time)
+        log = np.zeros(max(nsteps, 1))
+        done = C.c_int()
+        capi.check(self.L.mhd_run(self.h, nsteps, tmax, mode, C.byref(t), _dp(log), C.byref(done)))
+        return t.value, log[:done.value].copy(), done.value
+
+    def last_run_timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int()
+        capi.check(self.L.mhd_last_run_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- problems and diagnostics
+    def generate_problem_kh(self, vy_amp: float = 1.0e-3, rrv: float = 1.0e-2):
+        capi.check(self.L.mhd_generate_problem_kh(self.h, vy_amp, rrv))
+
+    def generate_problem_shear(self, rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1,
+                               seed=20261203):
+        p = capi.MhdIcParams(rho0, p0, v0, a, bx0, amp, sigma, seed)
+        capi.check(self.L.mhd_generate_problem_shear(self.h, C.byref(p)))
+
+    def diagnostics(self) -> np.ndarray:
+        """(volume, mix, sqrt(<vy^2>), max|divB|) of the current primitive state."""
+        out = np.zeros(4)
+        capi.check(self.L.mhd_diagnostics(self.h, _dp(out)))
+        return out
+
+    def status(self) -> int:
+        s = C.c_int()
+        capi.check(self.L.mhd_status(self.h, C.byref(s)))
+        return s.value
+
+    def interior(self, arr: np.ndarray) -> np.ndarray:
+        """(nv, nz, ny, nx) copy of the interior cells of a (ktot, jtot, nv, itot) array."""
+        return np.ascontiguousarray(arr[2:self.ktot - 3, 2:self.jtot - 3, :, 2:self.itot - 3].transpose(2, 0, 1, 3))
