@@ -1,0 +1,137 @@
+"""Benchmark: cell-updates/s of the MHD time step on one MI355X, with its HBM roofline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--mode 0|1] [--no-cpu]
+
+Workload: a 256^3 FP64 Kelvin-Helmholtz shear-layer box (gamma 5/3, Bx = 0.1,
+vx = 0.5 tanh(y/0.05), seeded vy kick; periodic x,z, reflecting y), advanced with the
+reference's loop body (MUSCL + van Leer, HLLD, GLM, one forward-Euler stage, CFL
+factor 0.05).  The state is generated on the device, so inputs are resident in HBM
+before the timed region.  One step = one loop body over all 256^3 interior cells.
+
+Printed JSON line: value = interior cells x steps / wall seconds of the timed region.
+roofline: algorithmic bytes of a cell update (read + write the 10 FP64 conserved
+variables = 160 B) per launch of the dominant kernel / that kernel's average launch
+duration from HIP events on the library's stream, against 8 TB/s.
+cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
+host's cores on 2 steps of the same 256^3 box.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+BYTES_PER_CELL = 2 * 10 * 8      # read + write the FP64 conserved state (NV = 10)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_box(n: int, mode_precision: int):
+    from mhdturbulence_amd import Box, Config
+    cfg = Config(nx=n, ny=n, nz=n, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+                 gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0, precision=mode_precision)
+    b = Box(cfg)
+    b.generate_grid()
+    b.generate_problem_shear(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1, seed=20261203)
+    return b
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    """oracle on this host's cores, same box, `steps` steps (after building its state)"""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    orc.lib().orc_set_threads(threads)
+    o = orc.Oracle(nx=n, ny=n, nz=n, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+                   gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0)
+    o.generate_grid()
+    o.generate_problem_shear(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1, seed=20261203)
+    o.run(1)  # first touch of every page outside the timed steps
+    t0 = time.perf_counter()
+    o.run(steps)
+    wall = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {n}^3 box (oracle C++/OpenMP restatement), {wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--mode", type=int, default=0, help="0 fused step kernels, 1 staged kernels")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 reconstruction + HLLD")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    log(f"[bench] building {n}^3 box on the device")
+    box = make_box(n, args.precision)
+    log(f"[bench] device bytes {box.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
+    t = 0.0
+    t, _, _ = box.run(args.warmup, time=t, mode=args.mode)
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    t, dts, done = box.run(args.steps, time=t, mode=args.mode)   # returns after the stream drained
+    wall = time.perf_counter() - t0
+    assert done == args.steps
+    ev_ms, launches = box.last_run_timing()
+    status = box.status()
+    value = n ** 3 * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    log(f"[bench] {value:.4e} cell-updates/s, {ms_per_step:.3f} ms/step (events {ev_ms / args.steps:.3f} ms/step)")
+
+    log(f"[bench] per-kernel profile over {args.profile_steps} steps")
+    prof = box.profile_kernels(args.profile_steps, mode=args.mode)
+    dominant = max(prof, key=prof.get)
+    dom_ms = prof[dominant]
+    step_ms = sum(prof.values())
+    # algorithmic bytes per launch: a launch of any step kernel advances the whole box once
+    alg_bytes = BYTES_PER_CELL * n ** 3
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                "kernel_ms": round(dom_ms, 4),
+                "step": {"achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2),
+                         "frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    out = {
+        "metric": "cell-updates/s (% HBM roofline) at 256^3 FP64",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f64 state, f32 reconstruction+HLLD",
+        "data": "synthetic (device-generated seeded shear-layer state)",
+        "config": {"workload": f"KH shear layer {n}^3, {'fused' if args.mode == 0 else 'staged'} step kernels",
+                   "cells": n ** 3, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": 0.05, "stages": 1,
+                   "energy_flux": "(v.B)B_n", "parallelism": "single GPU"},
+        "roofline": roofline,
+        "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
+        "status": status,
+        "sim_time": t,
+    }
+    del box
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {n}^3")
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

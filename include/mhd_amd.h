@@ -131,6 +131,14 @@ int mhd_status(mhd_ctx* ctx, int* status);
  * context's stream, measured with HIP events, and the launches they cover */
 int mhd_last_run_timing(mhd_ctx* ctx, double* kernel_ms, int* launches);
 
+/* Per-kernel timing (the phase timers of src_cpp_kokkos_device/main.cpp:293-373, as HIP
+ * events): runs nsteps loop iterations of the given mode launch by launch, with an event
+ * between launches, and writes the average ms of each launch position of an iteration
+ * into avg_ms (capacity cap); the state advances by nsteps steps. */
+int mhd_profile_kernels(mhd_ctx* ctx, int nsteps, int mode, double* avg_ms, int cap, int* nkernels);
+/* comma-separated names of the launch positions of one iteration of the given mode */
+const char* mhd_kernel_names(int mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ EXPORTS = (
     "mhd_control_timestep", "mhd_set_boundary", "mhd_evaluate_ch", "mhd_numerical_flux",
     "mhd_update_conserv", "mhd_damp_psi", "mhd_update_primitive", "mhd_run", "mhd_generate_problem_kh",
     "mhd_generate_problem_shear", "mhd_diagnostics", "mhd_status", "mhd_last_run_timing",
+    "mhd_profile_kernels", "mhd_kernel_names",
 )
 
 
@@ -86,6 +87,8 @@ def load(path: str = LIB_PATH):
         "mhd_diagnostics": (C.c_int, [vp, dp]),
         "mhd_status": (C.c_int, [vp, ip]),
         "mhd_last_run_timing": (C.c_int, [vp, dp, ip]),
+        "mhd_profile_kernels": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, ip]),
+        "mhd_kernel_names": (C.c_char_p, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -359,30 +359,52 @@ int mhd_update_primitive(mhd_ctx* c) {
   return MHD_OK;
 }
 
-// launches of one loop iteration; every argument is iteration-invariant
-static int issue_step(mhd_ctx* c, int mode, int* launches) {
+// Launches of one loop iteration; every argument is iteration-invariant (the kernels read
+// the iteration's slots from c->sc), so one iteration is captured once and replayed.
+// marks (may be null) receives an event after each launch, for per-kernel timing.
+static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
   const Geom& g = c->g;
+  int q = 0;
+  auto mark = [&]() {
+    if (marks) (void)hipEventRecord(marks[q], c->stream);
+    ++q;
+  };
   (void)mode;
-  {
-    mhd::launch_reduce_prim(g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
-    mhd::launch_boundary_prim(g, c->cfg.bc, c->P, c->stream);
-    for (int d = 1; d <= 3; ++d)
-      mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
-    mhd::launch_update(g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, 0.0, c->cfg.cfl,
-                       c->cfg.alphabp, c->sc, c->stream);
-    mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
-    mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, c->stream);
-    *launches = 8;
+  mhd::launch_reduce_prim(g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
+  mark();
+  mhd::launch_boundary_prim(g, c->cfg.bc, c->P, c->stream);
+  mark();
+  for (int d = 1; d <= 3; ++d) {
+    mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
+    mark();
   }
+  mhd::launch_update(g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, 0.0, c->cfg.cfl,
+                     c->cfg.alphabp, c->sc, c->stream);
+  mark();
+  mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
+  mark();
+  mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, c->stream);
+  mark();
   HIPCHK(hipGetLastError());
   return MHD_OK;
 }
 
-int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done) {
+static const char* kernel_names(int mode) {
+  (void)mode;
+  return "reduce_dt_ch,boundary,flux_x,flux_y,flux_z,update_damp,update_prim,step_end";
+}
+
+static int launches_per_step(int mode) {
+  (void)mode;
+  return 8;
+}
+
+// loop state and buffers for nsteps iterations starting at *time
+static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
   CHKRET(need_grid(c));
-  if (nsteps < 0 || !time) return fail(MHD_ERR_ARG, "nsteps >= 0 and a time pointer are required");
+  if (nsteps < 0) return fail(MHD_ERR_ARG, "nsteps must be >= 0");
   if (mode != 0 && mode != 1) return fail(MHD_ERR_ARG, "mode is 0 (fused) or 1 (staged)");
-  // device dt log, grown as needed (the captured graph holds its address)
+  // device dt log, grown as needed (a captured graph holds its address)
   if (nsteps > c->dt_log_cap) {
     drop_graph(c);
     if (c->dt_log) { (void)hipFree(c->dt_log); c->bytes -= c->dt_log_cap * sizeof(double); c->dt_log = nullptr; }
@@ -390,51 +412,88 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
     c->dt_log_cap = std::max(nsteps, 1);
   }
   CHKRET(ensure_flux(c));
-  // loop state
-  StepScalars h{};
-  h.time = *time;
-  h.tmax = tmax;
-  h.dt_log_cap = c->dt_log_cap;
-  HIPCHK(hipMemcpyAsync(&c->sc->time, &h.time, sizeof(double) * 2, hipMemcpyHostToDevice, c->stream));
-  {
-    int zeros[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(&c->sc->iter, zeros, sizeof(int) * 3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, &h.dt_log_cap, sizeof(int), hipMemcpyHostToDevice, c->stream));
-  }
+  double tt[2] = {time, tmax};
+  HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
+  int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
+  HIPCHK(hipMemcpyAsync(&c->sc->iter, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, &c->dt_log_cap, sizeof(int), hipMemcpyHostToDevice, c->stream));
   mhd::launch_reset_scalars(c->sc, 1, c->stream);
   HIPCHK(hipGetLastError());
+  return MHD_OK;
+}
+
+static int finish_run(mhd_ctx* c, int nsteps, double* time, double* dt_log, int* steps_done) {
+  StepScalars h;
+  CHKRET(read_scalars(c, &h));
+  if (dt_log && h.steps_done > 0)
+    HIPCHK(hipMemcpy(dt_log, c->dt_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
+  if (time) *time = h.time;
+  if (steps_done) *steps_done = h.steps_done;
+  return MHD_OK;
+}
+
+int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done) {
+  if (!time) return fail(MHD_ERR_ARG, "a time pointer is required");
+  CHKRET(prepare_run(c, nsteps, tmax, mode, *time));
   // capture one iteration once, then replay it
   if (!c->graph || c->graph_mode != mode || c->graph_log != c->dt_log) {
     drop_graph(c);
-    hipStream_t cap = c->stream;
-    HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    int launches = 0;
-    int rc = issue_step(c, mode, &launches);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = issue_step(c, mode, nullptr);
     hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(cap, &graph);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != MHD_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
     if (e != hipSuccess) return fail(MHD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) { c->graph = nullptr; return fail(MHD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    if (e != hipSuccess) {
+      c->graph = nullptr;
+      return fail(MHD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
     c->graph_mode = mode;
     c->graph_log = c->dt_log;
-    c->last_launches = launches;
   }
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
-  CHKRET(read_scalars(c, &h));
+  CHKRET(finish_run(c, nsteps, time, dt_log, steps_done));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
-  c->last_launches = c->last_launches * nsteps / std::max(nsteps, 1) * (nsteps > 0 ? 1 : 0);
-  if (dt_log && h.steps_done > 0)
-    HIPCHK(hipMemcpy(dt_log, c->dt_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
-  *time = h.time;
-  if (steps_done) *steps_done = h.steps_done;
+  c->last_launches = launches_per_step(mode) * nsteps;
   return MHD_OK;
 }
+
+int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int cap, int* nkernels) {
+  if (!c || !avg_ms || !nkernels || nsteps < 1) return fail(MHD_ERR_ARG, "bad profile arguments");
+  const int nk = launches_per_step(mode);
+  if (cap < nk) return fail(MHD_ERR_ARG, "avg_ms too small");
+  double t = 0.0;
+  CHKRET(prepare_run(c, nsteps, 0.0, mode, t));
+  std::vector<hipEvent_t> ev(nk + 1, nullptr);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  std::vector<double> acc(nk, 0.0);
+  int rc = MHD_OK;
+  for (int s = 0; s < nsteps && rc == MHD_OK; ++s) {
+    if (hipEventRecord(ev[0], c->stream) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventRecord"); break; }
+    rc = issue_step(c, mode, ev.data() + 1);
+    if (rc != MHD_OK) break;
+    if (hipEventSynchronize(ev[nk]) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventSynchronize"); break; }
+    for (int q = 0; q < nk; ++q) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+      acc[q] += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc != MHD_OK) return rc;
+  CHKRET(finish_run(c, nsteps, &t, nullptr, nullptr));
+  for (int q = 0; q < nk; ++q) avg_ms[q] = acc[q] / nsteps;
+  *nkernels = nk;
+  return MHD_OK;
+}
+
+const char* mhd_kernel_names(int mode) { return kernel_names(mode); }
 
 int mhd_generate_problem_kh(mhd_ctx* c, double vy_amp, double rrv) {
   CHKRET(need_grid(c));

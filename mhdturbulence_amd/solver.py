@@ -176,6 +176,14 @@ class Box:
         capi.check(self.L.mhd_run(self.h, nsteps, tmax, mode, C.byref(t), _dp(log), C.byref(done)))
         return t.value, log[:done.value].copy(), done.value
 
+    def profile_kernels(self, nsteps: int, mode: int = 0) -> dict:
+        """Average ms per launch position of one loop iteration (advances the state)."""
+        ms = np.zeros(32)
+        n = C.c_int()
+        capi.check(self.L.mhd_profile_kernels(self.h, nsteps, mode, _dp(ms), 32, C.byref(n)))
+        names = self.L.mhd_kernel_names(mode).decode().split(",")
+        return dict(zip(names, ms[:n.value].tolist()))
+
     def last_run_timing(self) -> tuple[float, int]:
         ms, n = C.c_double(), C.c_int()
         capi.check(self.L.mhd_last_run_timing(self.h, C.byref(ms), C.byref(n)))

@@ -1,0 +1,213 @@
+"""Parity of the HIP library (through the C ABI) with the CPU oracle on identical inputs.
+
+Bar: FP64 per-variable relative L1 <= 1e-10 after 100 steps, equal dt sequences, and
+each reference step function matching the oracle's to round-off.  Boxes are small
+enough for the oracle to finish in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mhdturbulence_amd import Box, Config
+
+pytestmark = pytest.mark.gpu
+
+KH_BC = (1, 1, 2, 2, 1, 1)
+TOL_L1 = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l1(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """per-variable relative L1 of (nv, ...) arrays; all-zero variables use an absolute mean."""
+    nv = a.shape[0]
+    a = a.reshape(nv, -1)
+    b = b.reshape(nv, -1)
+    num = np.abs(a - b).sum(1)
+    den = np.abs(b).sum(1)
+    return np.where(den > 1e-30 * b.shape[1], num / np.maximum(den, 1e-300), num / b.shape[1])
+
+
+def pair(orc, n=(32, 32, 32), problem="kh", upload=True, **kw):
+    """an oracle box and a GPU box with the same grid and state"""
+    o = orc.Oracle(nx=n[0], ny=n[1], nz=n[2], **{k: v for k, v in kw.items() if k != "precision"})
+    o.generate_grid()
+    if problem == "kh":
+        o.generate_problem_kh()
+    elif problem == "kh2":
+        o.generate_problem_kh(vy_amp=1e-2)
+    else:
+        o.generate_problem_shear(**problem)
+    b = Box(Config(nx=n[0], ny=n[1], nz=n[2], **kw))
+    b.set_grid(o.grid(1), o.grid(1, True), o.grid(2), o.grid(2, True), o.grid(3), o.grid(3, True))
+    if upload:
+        b.upload_prim(o.P)
+        b.upload_cons(o.U)
+    return o, b
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def test_device_grid_is_bitwise_the_oracle_grid(orc):
+    for shift in (1, 0):
+        say("grid shift", shift)
+        o = orc.Oracle(nx=40, ny=24, nz=16, grid_shift=shift)
+        o.generate_grid()
+        b = Box(Config(nx=40, ny=24, nz=16, grid_shift=shift))
+        b.generate_grid()
+        got = b.get_grid()
+        ref = [o.grid(1), o.grid(1, True), o.grid(2), o.grid(2, True), o.grid(3), o.grid(3, True)]
+        for g, r, n in zip(got, ref, (41, 41, 25, 25, 17, 17)):
+            np.testing.assert_array_equal(g[:n + 3], r[:n + 3])
+
+
+def test_upload_download_roundtrip(orc):
+    o, b = pair(orc, n=(12, 10, 8))
+    np.testing.assert_array_equal(b.download_prim(), o.P)
+    np.testing.assert_array_equal(b.download_cons(), o.U)
+
+
+@pytest.mark.parametrize("problem", ["kh", "shear"])
+def test_device_initial_state(orc, problem):
+    say("device IC", problem)
+    o, b = pair(orc, n=(24, 20, 16), problem="kh" if problem == "kh" else dict(bx0=0.1, a=0.05, seed=7),
+                upload=False, gam=5.0 / 3.0 if problem == "shear" else 1.4)
+    if problem == "kh":
+        b.generate_problem_kh()
+    else:
+        b.generate_problem_shear(bx0=0.1, a=0.05, seed=7)
+    # libm vs device math differ in the last bits of tanh/exp/sin/cos only
+    np.testing.assert_allclose(o.interior(b.download_prim()), o.interior(o.P), rtol=0, atol=2e-15)
+    np.testing.assert_allclose(o.interior(b.download_cons()), o.interior(o.U), rtol=0, atol=5e-15)
+
+
+@pytest.mark.parametrize("eflux,problem", [(0, "kh2"), (1, "shear"), (0, "shear")])
+def test_reference_functions_one_by_one(orc, eflux, problem):
+    say("one-by-one", eflux, problem)
+    prob = problem if problem != "shear" else dict(bx0=0.2, a=0.08, amp=0.05)
+    o, b = pair(orc, n=(20, 18, 16), problem=prob, energy_flux=eflux)
+    dt_o, dt_g = o.control_timestep(), b.control_timestep()
+    assert dt_g == pytest.approx(dt_o, rel=1e-15)
+    o.set_boundary()
+    b.set_boundary_condition()
+    Pg = b.download_prim()
+    # every ghost the step reads: x ghosts at interior (j,k), y at interior (i,k), z at interior (i,j)
+    for sl in (np.s_[2:-3, 2:-3, :, 0:2], np.s_[2:-3, 2:-3, :, -3:-1], np.s_[2:-3, 0:2, :, 2:-3],
+               np.s_[2:-3, -3:-1, :, 2:-3], np.s_[0:2, 2:-3, :, 2:-3], np.s_[-3:-1, 2:-3, :, 2:-3]):
+        np.testing.assert_array_equal(Pg[sl], o.P[sl])
+    ch_o, ch_g = o.evaluate_ch(), b.evaluate_ch()
+    assert ch_g == pytest.approx(ch_o, rel=1e-15)
+    for d in (1, 2, 3):
+        o.numerical_flux(d, ch_o)
+        b.get_numerical_flux(d, ch_o)
+        Fo, Fg = o.F(d), b.download_flux(d)
+        sl = [slice(2, -3)] * 3
+        sl[2 - (d - 1)] = slice(2, -2)  # faces run one further along the sweep axis
+        s = (sl[0], sl[1], slice(None), sl[2])
+        scale = np.abs(Fo[s]).max(axis=(0, 1, 3)) + 1e-12
+        err = (np.abs(Fg[s] - Fo[s]).max(axis=(0, 1, 3))) / scale
+        assert np.all(err < 1e-12), (d, err)
+    o.update_conserv(dt_o)
+    b.update_conserv_u(dt_o)
+    o.damp_psi(dt_o, ch_o)
+    b.damp_psi(dt_o, ch_o)
+    assert np.all(rel_l1(o.interior(b.download_cons()), o.interior(o.U)) < 1e-14)
+    o.update_primitive()
+    b.update_primitiv_p()
+    assert np.all(rel_l1(o.interior(b.download_prim()), o.interior(o.P)) < 1e-14)
+
+
+def _run_parity(orc, n, steps, mode, problem="kh", **kw):
+    o, b = pair(orc, n=n, problem=problem, **kw)
+    t_o, log_o = o.run(steps)
+    t_g, log_g, done = b.run(steps, mode=mode)
+    assert done == steps
+    np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+    assert t_g == pytest.approx(t_o, rel=1e-12)
+    errP = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+    errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+    say(f"n={n} steps={steps} mode={mode} max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+    return errP, errU, b, o
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_kh_64cube_100_steps(orc, mode):
+    say("KH 64^3 100 steps mode", mode)
+    errP, errU, _, _ = _run_parity(orc, (64, 64, 64), 100, mode)
+    assert np.all(errP <= TOL_L1), errP
+    assert np.all(errU <= TOL_L1), errU
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("eflux", [0, 1])
+def test_magnetised_anisotropic(orc, mode, eflux):
+    say("magnetised 40x24x16 mode", mode, "eflux", eflux)
+    errP, errU, _, _ = _run_parity(orc, (40, 24, 16), 60, mode, problem=dict(bx0=0.15, a=0.05, amp=0.02),
+                                   energy_flux=eflux, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+                                   grid_shift=0)
+    assert np.all(errP <= TOL_L1), errP
+    assert np.all(errU <= TOL_L1), errU
+
+
+@pytest.mark.parametrize("bc", [(3, 3, 2, 2, 3, 3), (2, 2, 3, 3, 1, 1)])
+def test_other_boundaries(orc, bc):
+    say("boundaries", bc)
+    for mode in (1, 0):
+        errP, errU, _, _ = _run_parity(orc, (24, 24, 20), 40, mode, problem=dict(bx0=0.1, a=0.06, amp=0.05), bc=bc,
+                                       energy_flux=1)
+        assert np.all(errP <= TOL_L1), (mode, errP)
+
+
+def test_tmax_stop_matches_reference_loop(orc):
+    o, b = pair(orc, n=(16, 16, 16))
+    dt0 = o.control_timestep()
+    tmax = 5.5 * dt0
+    t, log, done = b.run(50, tmax=tmax, mode=0)
+    # the reference loop breaks after the step that takes time past time_max
+    assert done == 6 and t > tmax and t - log[-1] <= tmax
+    t_o, log_o = o.run(done)
+    np.testing.assert_allclose(log, log_o, rtol=1e-12)
+
+
+def test_run_continues_across_calls(orc):
+    o, b = pair(orc, n=(24, 24, 24))
+    t1, l1, _ = b.run(7, mode=0)
+    t2, l2, _ = b.run(8, time=t1, mode=0)
+    t_o, log_o = o.run(15)
+    np.testing.assert_allclose(np.concatenate([l1, l2]), log_o, rtol=1e-12)
+    assert np.all(rel_l1(o.interior(b.download_cons()), o.interior(o.U)) <= TOL_L1)
+
+
+def test_diagnostics_match(orc):
+    o, b = pair(orc, n=(32, 32, 16), problem=dict(bx0=0.1, a=0.05, amp=0.03), energy_flux=1)
+    o.run(20)
+    b.run(20, mode=0)
+    do, dg = o.diagnostics(), b.diagnostics()
+    np.testing.assert_allclose(dg[:3], do[:3], rtol=1e-12)
+    assert dg[3] == pytest.approx(do[3], rel=1e-8, abs=1e-14)
+
+
+def test_mixed_precision_close_to_fp64(orc):
+    say("mixed precision")
+    o, b = pair(orc, n=(32, 32, 32), problem=dict(bx0=0.1, a=0.05, amp=0.02), energy_flux=1, gam=5.0 / 3.0,
+                precision=1)
+    o.run(50)
+    b.run(50, mode=0)
+    err = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+    say("mixed relL1", err)
+    assert err[0] < 1e-6 and err[5] < 1e-6
+    assert np.all(err[[0, 1, 5, 7, 11]] < 1e-4)
+
+
+def test_kernel_profile_runs(orc):
+    o, b = pair(orc, n=(32, 32, 32))
+    for mode in (1, 0):
+        prof = b.profile_kernels(3, mode=mode)
+        assert len(prof) >= 2 and all(v >= 0 for v in prof.values())
