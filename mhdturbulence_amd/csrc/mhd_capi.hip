@@ -12,6 +12,7 @@
 
 #include "../../include/mhd_amd.h"
 #include "mhd_device.hpp"
+#include "mhd_fused.hpp"
 
 using mhd::Geom;
 using mhd::StepScalars;
@@ -64,6 +65,11 @@ struct mhd_ctx {
   hipGraphExec_t graph = nullptr;
   int graph_mode = -1;
   double* graph_log = nullptr;
+  // fused loop buffers: second conserved copy (U2 above), flux divergence, strip-edge fluxes
+  double* acc = nullptr;
+  double* edge = nullptr;
+  mhd::FusedArgs fa{};
+  bool fa_ready = false;
 };
 
 namespace {
@@ -84,6 +90,32 @@ int dalloc(mhd_ctx* c, double** p, size_t n) {
 
 int ensure_flux(mhd_ctx* c) {
   for (int d = 0; d < 3; ++d) CHKRET(dalloc(c, &c->F[d], mhd::NCONS * c->g.vstride));
+  return MHD_OK;
+}
+
+int ensure_fused(mhd_ctx* c) {
+  if (c->fa_ready) return MHD_OK;
+  mhd::FusedArgs& a = c->fa;
+  a.g = c->g;
+  mhd::fused_configure(a);
+  CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
+  CHKRET(dalloc(c, &c->acc, mhd::NCONS * c->g.vstride));
+  CHKRET(dalloc(c, &c->edge, mhd::fused_edge_elems(a)));
+  a.x1a = c->grid[0];
+  a.x2a = c->grid[2];
+  a.x3a = c->grid[4];
+  a.U0 = c->U;
+  a.U1 = c->U2;
+  a.acc = c->acc;
+  a.edge = c->edge;
+  a.sc = c->sc;
+  a.gam = c->cfg.gam;
+  a.cfl = c->cfg.cfl;
+  a.alphabp = c->cfg.alphabp;
+  for (int q = 0; q < 6; ++q) a.bc[q] = c->cfg.bc[q];
+  a.eflux = c->cfg.energy_flux;
+  a.mixed = c->cfg.precision;
+  c->fa_ready = true;
   return MHD_OK;
 }
 
@@ -153,7 +185,7 @@ int mhd_destroy(mhd_ctx* c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->staging, c->diag, c->dt_log,
+  double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->acc, c->edge, c->staging, c->diag, c->dt_log,
                     c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5]};
   for (double* b : bufs)
     if (b) hipFree(b);
@@ -316,7 +348,7 @@ int mhd_evaluate_ch(mhd_ctx* c, double* chg) {
 int mhd_set_boundary(mhd_ctx* c) {
   if (!c) return fail(MHD_ERR_ARG, "null context");
   CHKRET(set_dev(c));
-  mhd::launch_boundary_prim(c->g, c->cfg.bc, c->P, c->stream);
+  mhd::launch_boundary(c->g, c->cfg.bc, c->P, mhd::NPRIM, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return MHD_OK;
@@ -369,10 +401,21 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
     if (marks) (void)hipEventRecord(marks[q], c->stream);
     ++q;
   };
-  (void)mode;
+  if (mode == 0) {
+    mhd::launch_fused_edge(c->fa, c->stream);
+    mark();
+    mhd::launch_fused_xy(c->fa, c->stream);
+    mark();
+    mhd::launch_fused_z(c->fa, c->stream);
+    mark();
+    mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 1, c->stream);
+    mark();
+    HIPCHK(hipGetLastError());
+    return MHD_OK;
+  }
   mhd::launch_reduce_prim(g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
   mark();
-  mhd::launch_boundary_prim(g, c->cfg.bc, c->P, c->stream);
+  mhd::launch_boundary(g, c->cfg.bc, c->P, mhd::NPRIM, c->stream);
   mark();
   for (int d = 1; d <= 3; ++d) {
     mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
@@ -383,21 +426,18 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
   mark();
   mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
   mark();
-  mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, c->stream);
+  mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 0, c->stream);
   mark();
   HIPCHK(hipGetLastError());
   return MHD_OK;
 }
 
 static const char* kernel_names(int mode) {
-  (void)mode;
+  if (mode == 0) return "fused_edge_x,fused_xy,fused_z_update,step_end";
   return "reduce_dt_ch,boundary,flux_x,flux_y,flux_z,update_damp,update_prim,step_end";
 }
 
-static int launches_per_step(int mode) {
-  (void)mode;
-  return 8;
-}
+static int launches_per_step(int mode) { return mode == 0 ? 4 : 8; }
 
 // loop state and buffers for nsteps iterations starting at *time
 static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
@@ -411,20 +451,36 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
     CHKRET(dalloc(c, &c->dt_log, std::max(nsteps, 1)));
     c->dt_log_cap = std::max(nsteps, 1);
   }
-  CHKRET(ensure_flux(c));
+  if (mode == 0) CHKRET(ensure_fused(c));
+  else CHKRET(ensure_flux(c));
   double tt[2] = {time, tmax};
   HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
   int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
   HIPCHK(hipMemcpyAsync(&c->sc->iter, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, &c->dt_log_cap, sizeof(int), hipMemcpyHostToDevice, c->stream));
   mhd::launch_reset_scalars(c->sc, 1, c->stream);
+  if (mode == 0) {
+    // the first iteration's dt and c_h from P (ControlTimestep / EvaluateCh on the state as
+    // given), and the ghost cells of U, which the fused kernels read
+    mhd::launch_reduce_prim(c->g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
+    mhd::launch_boundary(c->g, c->cfg.bc, c->U, mhd::NCONS, c->stream);
+  }
   HIPCHK(hipGetLastError());
   return MHD_OK;
 }
 
-static int finish_run(mhd_ctx* c, int nsteps, double* time, double* dt_log, int* steps_done) {
+static int finish_run(mhd_ctx* c, int mode, int nsteps, double* time, double* dt_log, int* steps_done) {
   StepScalars h;
   CHKRET(read_scalars(c, &h));
+  if (mode == 0) {
+    // the state sits in U2 after an odd number of iterations; P is recovered from it
+    if (h.iter & 1)
+      HIPCHK(hipMemcpyAsync(c->U, c->U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+    mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   if (dt_log && h.steps_done > 0)
     HIPCHK(hipMemcpy(dt_log, c->dt_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
   if (time) *time = h.time;
@@ -456,7 +512,7 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
-  CHKRET(finish_run(c, nsteps, time, dt_log, steps_done));
+  CHKRET(finish_run(c, mode, nsteps, time, dt_log, steps_done));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
@@ -487,7 +543,7 @@ int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int ca
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc != MHD_OK) return rc;
-  CHKRET(finish_run(c, nsteps, &t, nullptr, nullptr));
+  CHKRET(finish_run(c, mode, nsteps, &t, nullptr, nullptr));
   for (int q = 0; q < nk; ++q) avg_ms[q] = acc[q] / nsteps;
   *nkernels = nk;
   return MHD_OK;

@@ -70,7 +70,8 @@ void launch_reset_scalars(StepScalars* sc, int ring, hipStream_t s);
 // ring=1: into sc->red[sc->iter % 3].
 void launch_reduce_prim(const Geom& g, const double* P, const double* x1a, const double* x2a, const double* x3a,
                         StepScalars* sc, int ring, hipStream_t s);
-void launch_boundary_prim(const Geom& g, const int* bc, double* P, hipStream_t s);
+// ghost fill of an nv-variable array (P: nv = 12, U: nv = 10)
+void launch_boundary(const Geom& g, const int* bc, double* A, int nv, hipStream_t s);
 // chg: value, or (sc != nullptr) taken from sc->red[sc->iter % 3][1]
 void launch_flux(const Geom& g, int dir, int eflux, int mixed, const double* P, double* F, double chg,
                  const StepScalars* sc, hipStream_t s);
@@ -83,8 +84,9 @@ void launch_damp(const Geom& g, const double* x1a, const double* x2a, const doub
                  double chg, double alphabp, hipStream_t s);
 void launch_update_prim(const Geom& g, double gam, const double* U, double* P, const StepScalars* sc,
                         hipStream_t s);
-// time += dt, log dt, iter += 1, reset the next ring slot, stop once time > tmax
-void launch_step_end(StepScalars* sc, double cfl, double* dt_log, hipStream_t s);
+// time += dt, log dt, iter += 1, reset the ring slot the next iteration accumulates into
+// (iter + ahead) % 3, stop once time > tmax
+void launch_step_end(StepScalars* sc, double cfl, double* dt_log, int ahead, hipStream_t s);
 // per-k-plane partial sums: partial[4*kk + {0,1,2,3}] = (vol, mix, vy^2, max|divB|)
 void launch_diagnostics(const Geom& g, const double* P, const double* x1a, const double* x1b,
                         const double* x2a, const double* x2b, const double* x3a, const double* x3b,

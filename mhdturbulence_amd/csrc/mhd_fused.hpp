@@ -1,0 +1,46 @@
+// mhd_fused.hpp -- the fused step: three kernels per loop iteration instead of eight.
+//
+//   edge : x-face fluxes on the boundaries between x-strips (one thread per face)
+//   xy   : x- and y-face fluxes and their divergence  acc = dFx/dx + dFy/dy
+//          (a wavefront holds a 64/32/16-cell x-strip in its lanes; x neighbours come
+//          through cross-lane shuffles; it marches in y keeping a 3-row register window)
+//   z    : z-face fluxes marching in z, then U_new = U - dt*(acc + dFz/dz), psi damping,
+//          primitive recovery of U_new, the next iteration's dt / c_h reduction, and the
+//          ghost cells of U_new (periodic / reflecting / outflow images)
+//
+// Only the 10 conserved variables stay resident (two ping-pong copies); primitives are
+// recovered from U on load, as UpdatePrimitvP (src_cpp_kokkos_host/mhd.cpp:1447-1509)
+// would have stored them.
+#pragma once
+
+#include "mhd_device.hpp"
+
+namespace mhd {
+
+struct FusedArgs {
+  Geom g;
+  const double* x1a;
+  const double* x2a;
+  const double* x3a;
+  double* U0;      // iteration iter reads U[iter & 1] and writes the other copy
+  double* U1;
+  double* acc;     // NCONS * vstride
+  double* edge;    // NCONS * nz * ny * (ns + 1)
+  StepScalars* sc;
+  double gam, cfl, alphabp;
+  int bc[6];
+  int sw;          // x-strip width of the xy kernel (64, 32 or 16)
+  int ns;          // strips per x-row
+  int ly, lz;      // y-chunk and z-chunk lengths
+  int eflux, mixed;
+};
+
+// strip width, chunk lengths and buffer sizes for a geometry
+void fused_configure(FusedArgs& a);
+size_t fused_edge_elems(const FusedArgs& a);
+
+void launch_fused_edge(const FusedArgs& a, hipStream_t s);
+void launch_fused_xy(const FusedArgs& a, hipStream_t s);
+void launch_fused_z(const FusedArgs& a, hipStream_t s);
+
+}  // namespace mhd

@@ -232,8 +232,9 @@ __device__ __forceinline__ int ghost_source(int bc_in, int bc_out, int side, int
   return side ? hi : lo;
 }
 
-__global__ __launch_bounds__(BLK) void k_boundary_prim(Geom g, int bx0, int bx1, int by0, int by1, int bz0, int bz1,
-                                                       double* P) {
+// nv variables; the normal velocity (P) or momentum (U) sits at index 1 + axis in both.
+__global__ __launch_bounds__(BLK) void k_boundary(Geom g, int nv, int bx0, int bx1, int by0, int by1, int bz0, int bz1,
+                                                  double* P) {
   const long long nxs = 4LL * g.ny * g.nz, nys = 4LL * g.nx * g.nz, nzs = 4LL * g.nx * g.ny;
   long long t = (long long)blockIdx.x * BLK + threadIdx.x;
   int dir, a, b, side, gg;
@@ -264,8 +265,7 @@ __global__ __launch_bounds__(BLK) void k_boundary_prim(Geom g, int bx0, int bx1,
     const int d = side ? NGH + g.nz + gg : gg;
     dst = g.cell(d, j, i); src = g.cell(s, j, i); nneg = nve3;
   }
-#pragma unroll
-  for (int n = 0; n < NPRIM; ++n) {
+  for (int n = 0; n < nv; ++n) {
     const double v = P[n * g.vstride + src];
     P[n * g.vstride + dst] = (neg && n == nneg) ? -v : v;
   }
@@ -367,19 +367,20 @@ __global__ __launch_bounds__(BLK) void k_update_prim(Geom g, double gam, const d
   for (int n = 0; n < NPRIM; ++n) P[n * g.vstride + c] = q[n];
 }
 
-__global__ void k_step_end(StepScalars* sc, double cfl, double* dt_log) {
+// ahead = 0: the next iteration reduces into slot iter % 3 (staged loop);
+// ahead = 1: the next iteration reads slot iter % 3 and accumulates into (iter+1) % 3 (fused loop).
+__global__ void k_step_end(StepScalars* sc, double cfl, double* dt_log, int ahead) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (!sc->stopped) {
-    const double dt = cfl * __longlong_as_double((long long)sc->red[sc->iter % 3][0]);
-    if (dt_log && sc->steps_done < sc->dt_log_cap) dt_log[sc->steps_done] = dt;
-    sc->time += dt;
-    sc->steps_done += 1;
-    if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
-  }
+  if (sc->stopped) return;
+  const double dt = cfl * __longlong_as_double((long long)sc->red[sc->iter % 3][0]);
+  if (dt_log && sc->steps_done < sc->dt_log_cap) dt_log[sc->steps_done] = dt;
+  sc->time += dt;
+  sc->steps_done += 1;
   sc->iter += 1;
-  unsigned long long* nxt = sc->red[sc->iter % 3];
+  unsigned long long* nxt = sc->red[(sc->iter + ahead) % 3];
   nxt[0] = DT_INIT_BITS;
   nxt[1] = CH_INIT_BITS;
+  if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -474,10 +475,10 @@ void launch_reduce_prim(const Geom& g, const double* P, const double* x1a, const
   hipLaunchKernelGGL(k_reduce_prim, dim3(nb), dim3(BLK), 0, s, g, P, x1a, x2a, x3a, sc, ring);
 }
 
-void launch_boundary_prim(const Geom& g, const int* bc, double* P, hipStream_t s) {
+void launch_boundary(const Geom& g, const int* bc, double* A, int nv, hipStream_t s) {
   const long long n = 4LL * ((long long)g.ny * g.nz + (long long)g.nx * g.nz + (long long)g.nx * g.ny);
-  hipLaunchKernelGGL(k_boundary_prim, dim3(cdiv(n, BLK)), dim3(BLK), 0, s, g, bc[0], bc[1], bc[2], bc[3], bc[4],
-                     bc[5], P);
+  hipLaunchKernelGGL(k_boundary, dim3(cdiv(n, BLK)), dim3(BLK), 0, s, g, nv, bc[0], bc[1], bc[2], bc[3], bc[4],
+                     bc[5], A);
 }
 
 template <int D, int E, typename R>
@@ -524,8 +525,8 @@ void launch_update_prim(const Geom& g, double gam, const double* U, double* P, c
   hipLaunchKernelGGL(k_update_prim, grid, dim3(BLK), 0, s, g, gam, U, P, sc);
 }
 
-void launch_step_end(StepScalars* sc, double cfl, double* dt_log, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, sc, cfl, dt_log);
+void launch_step_end(StepScalars* sc, double cfl, double* dt_log, int ahead, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, sc, cfl, dt_log, ahead);
 }
 
 void launch_diagnostics(const Geom& g, const double* P, const double* x1a, const double* x1b, const double* x2a,
