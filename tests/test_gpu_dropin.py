@@ -1,0 +1,43 @@
+"""The C++ drop-in layer (mhdturbulence_amd/host: hydflux_mod / boundary_mod over the C ABI)
+driving the src_cpp_kokkos_host call sequence on the 150^3 Kelvin-Helmholtz problem,
+compared with the oracle after a few steps."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import parity
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mhdturbulence_amd", "mhd_kh_main")
+N = 150
+STEPS = 4
+
+
+@pytest.fixture(scope="module")
+def oracle_state(oracle_lib):
+    o = oracle_lib.Oracle(nx=N, ny=N, nz=N)
+    o.generate_grid()
+    o.generate_problem_kh(vy_amp=1e-3, rrv=1e-2)
+    t, log = o.run(STEPS)
+    return o, t, log
+
+
+@pytest.mark.parametrize("flag", [[], ["--fused"]])
+def test_kh_main_matches_oracle(tmp_path, oracle_state, flag):
+    o, t, log = oracle_state
+    dump = tmp_path / "P.bin"
+    r = subprocess.run([EXE, "--steps", str(STEPS), "--dump", str(dump), *flag], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    P = np.fromfile(dump, dtype=np.float64).reshape(o.ktot, o.jtot, 12, o.itot)
+    err = parity.rel_l1(o.interior(P), o.interior(o.P), "prim")
+    assert np.all(err <= 1e-10), err
+    # dt.log carries the first dt in %.17g
+    first = open(tmp_path / "dt.log").readline()
+    dt0 = float(first.split("dt=")[1])
+    assert dt0 == pytest.approx(log[0], rel=1e-14)
