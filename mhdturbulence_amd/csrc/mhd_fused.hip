@@ -21,9 +21,18 @@ constexpr int WAVES = 4;  // wavefronts per block; blocks carry independent wave
 
 // FP64 march kernels need more than 256 VGPRs; one wave per SIMD lets them keep their
 // window in AGPRs instead of scratch.  The FP32 variants fit two waves per SIMD.
+#ifndef MHD_F64_MINWAVES
+#define MHD_F64_MINWAVES 1
+#endif
+// MHD_PREFETCH=1: the march loads the raw state of the row two ahead, and the z kernel
+// issues the loads of the cell it updates before that iteration's Riemann solve, so the
+// load latency overlaps arithmetic even at one wavefront per SIMD.
+#ifndef MHD_PREFETCH
+#define MHD_PREFETCH 1
+#endif
 template <typename R>
 struct MinWaves {
-  static constexpr int value = sizeof(R) == 8 ? 1 : 2;
+  static constexpr int value = sizeof(R) == 8 ? MHD_F64_MINWAVES : 2;
 };
 
 __device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -35,6 +44,19 @@ __device__ __forceinline__ void load_prim(const double* __restrict__ U, size_t v
   double u[NCONS], p[NPRIM];
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) u[m] = U[m * vs + c];
+  cons_to_prim(u, gam, p);
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) q[n] = R(p[n]);
+}
+
+__device__ __forceinline__ void load_raw(const double* __restrict__ U, size_t vs, int c, double* u) {
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) u[m] = U[m * vs + c];
+}
+
+template <typename R>
+__device__ __forceinline__ void recover(const double* u, double gam, R* q) {
+  double p[NPRIM];
   cons_to_prim(u, gam, p);
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) q[n] = R(p[n]);
@@ -107,7 +129,7 @@ __global__ __launch_bounds__(256) void k_fused_edge(FusedArgs a) {
   cell_states<R>(q[0], q[1], q[2], ql, tmp);
   cell_states<R>(q[1], q[2], q[3], tmp, qr);
   double F[NCONS];
-  riemann_flux<0, EFLUX, R>(ql, qr, chg, F);
+  riemann_flux<0, EFLUX, R>(ql, qr, chg, 0.5 / chg, F);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) a.edge[edge_index(a, m, k, j, e)] = F[m];
 }
@@ -122,6 +144,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   const int iter = sc->iter;
   const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
   const double chg = bits2d(sc->red[iter % 3][1]);
+  const double hchg = 0.5 / chg;
   constexpr int SPW = 64 / SW;
   const int lane = threadIdx.x & 63;
   const int sub = lane / SW, l = lane - sub * SW;
@@ -158,9 +181,18 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fyprev[m] = 0.0;
+#if MHD_PREFETCH
+  double un[NCONS];
+  load_raw(U, vs, g.cell(k, j0, i), un);
+#endif
 
   for (int j = j0 - 1; j <= j1; ++j) {
+#if MHD_PREFETCH
+    recover<R>(un, a.gam, Pp);
+    load_raw(U, vs, g.cell(k, j + 2, i), un);  // row j+2 <= ny+4 < jtot
+#else
     load_prim<R>(U, vs, g.cell(k, j + 1, i), a.gam, Pp);
+#endif
     const int row = j - 1;  // the cell row completed in this pass
     const bool out_row = row >= j0;
     double ax[NCONS];
@@ -183,7 +215,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) rtl[n] = lane_get(rt[n], src_l);
       double Fl[NCONS];
-      riemann_flux<0, EFLUX, R>(rtl, lt, chg, Fl);
+      riemann_flux<0, EFLUX, R>(rtl, lt, chg, hchg, Fl);
       if (is_left) {
 #pragma unroll
         for (int m = 0; m < NCONS; ++m) Fl[m] = a.edge[edge_index(a, m, k, row, s)];
@@ -202,7 +234,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
     R up[NPRIM], dn[NPRIM];
     cell_states<R>(Pm, P0, Pp, up, dn);
     double Fy[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j >= j0) riemann_flux<1, EFLUX, R>(upprev, dn, chg, Fy);
+    if (j >= j0) riemann_flux<1, EFLUX, R>(upprev, dn, chg, hchg, Fy);
     if (out_row && active) {
       const double yinv = 1.0 / (a.x2a[row + 1] - a.x2a[row]);
       const int c = g.cell(k, row, i);
@@ -233,6 +265,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
   const unsigned long long* rd = sc->red[iter % 3];
   const double dt = a.cfl * bits2d(rd[0]);
   const double chg = bits2d(rd[1]);
+  const double hchg = 0.5 / chg;
   const int lane = threadIdx.x & 63;
   const int w = blockIdx.x * WAVES + (threadIdx.x >> 6);
   const int nxs = (g.nx + 63) / 64;
@@ -260,23 +293,47 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fzprev[m] = 0.0;
   double dtl = __builtin_inf(), chl = 0.0;
+#if MHD_PREFETCH
+  double un[NCONS];
+  load_raw(U, vs, g.cell(k0, j, i), un);
+#endif
 
   for (int k = k0 - 1; k <= k1; ++k) {
+    const int kk = k - 1;  // the cell updated in this pass
+    const bool upd = kk >= k0 && active;
+#if MHD_PREFETCH
+    recover<R>(un, a.gam, Pp);
+    load_raw(U, vs, g.cell(k + 2, j, i), un);  // plane k+2 <= nz+4 < ktot
+    double uo[NCONS], ac[NCONS];
+    if (upd) {
+      const int c = g.cell(kk, j, i);
+#pragma unroll
+      for (int m = 0; m < NCONS; ++m) {
+        uo[m] = U[m * vs + c];
+        ac[m] = a.acc[m * vs + c];
+      }
+    }
+#else
     load_prim<R>(U, vs, g.cell(k + 1, j, i), a.gam, Pp);
+#endif
     R up[NPRIM], dn[NPRIM];
     cell_states<R>(Pm, P0, Pp, up, dn);
     double Fz[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, Fz);
-    const int kk = k - 1;
-    if (kk >= k0 && active) {
+    if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, hchg, Fz);
+    if (upd) {
       const double dza = a.x3a[kk + 1] - a.x3a[kk];
       const double zinv = 1.0 / dza;
       const int c = g.cell(kk, j, i);
       double u[NCONS];
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) {
+#if MHD_PREFETCH
+        const double div = ac[m] + (Fz[m] - Fzprev[m]) * zinv;
+        u[m] = uo[m] - dt * div;
+#else
         const double div = a.acc[m * vs + c] + (Fz[m] - Fzprev[m]) * zinv;
         u[m] = U[m * vs + c] - dt * div;
+#endif
       }
       // DampPsi (KH/mhd.cpp:1625-1631)
       const double dhl = fmin(dxa, fmin(dya, dza));

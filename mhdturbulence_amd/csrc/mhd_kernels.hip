@@ -294,7 +294,7 @@ __global__ __launch_bounds__(BLK) void k_flux(Geom g, const double* __restrict__
   cell_states<R>(q[0], q[1], q[2], ql, tmp);  // left state of this face from cell f-1
   cell_states<R>(q[1], q[2], q[3], tmp, qr);  // right state from cell f
   double Fo[NCONS];
-  riemann_flux<D, EFLUX, R>(ql, qr, chg, Fo);
+  riemann_flux<D, EFLUX, R>(ql, qr, chg, 0.5 / chg, Fo);
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) F[n * g.vstride + c] = Fo[n];
 }

@@ -24,11 +24,38 @@ __host__ __device__ constexpr int slot(int c, int d) { return (c - d + 3) % 3; }
 // Cartesian component sitting in rotated slot s for direction D
 __host__ __device__ constexpr int comp(int s, int d) { return (s + d) % 3; }
 
+// Division of the hot paths.  MHD_FAST_DIV=1 (default): FP64 uses the hardware reciprocal
+// refined by two Newton steps and one residual correction (error within an ulp or two of
+// the IEEE quotient, no special-case scaling); FP32 uses the hardware reciprocal.
+// Operands are finite and bounded away from zero wherever the step divides (densities,
+// wave-speed differences guarded by +-1e-20, sums of same-sign slopes).
+#ifndef MHD_FAST_DIV
+#define MHD_FAST_DIV 1
+#endif
+template <typename R>
+__device__ __forceinline__ R dv(R a, R b) {
+  return a / b;
+}
+#if MHD_FAST_DIV
+template <>
+__device__ __forceinline__ double dv<double>(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+template <>
+__device__ __forceinline__ float dv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+#endif
+
 // van Leer harmonic limiter, KH/mhd.cpp:89-95
 template <typename R>
 __device__ __forceinline__ R van_leer(R dp, R dm) {
   const R prod = dp * dm;
-  const R v = R(2) * dp * dm / (dp + dm);
+  const R v = dv(R(2) * dp * dm, dp + dm);
   return prod > R(0) ? v : R(0);
 }
 
@@ -39,6 +66,7 @@ struct RState {
   R rho, mu, mv, mw, E, bu, bv, bw, psi, sc;    // rotated conserved
   R fd, fu, fv, fw, fE, fbv, fbw, fsc;           // rotated physical flux (fbu = fpsi = 0)
   R cf, vn, pt;                                  // fast speed, normal velocity, total pressure
+  R vv, vw, X;                                   // transverse velocities and passive scalar
 };
 
 // Prim2ConsD for direction D (KH/mhd.cpp:997-1076).  EFLUX=0: energy flux magnetic term
@@ -47,7 +75,7 @@ template <int D, int EFLUX, typename R>
 __device__ __forceinline__ void prim_to_state(const R* q, RState<R>& s) {
   constexpr int DF = (D + 1) % 3, DB = (D + 2) % 3;
   const R rho = q[nden];
-  const R rhoinv = R(1) / rho;
+  const R rhoinv = dv(R(1), rho);
   const R vsq = q[nve1] * q[nve1] + q[nve2] * q[nve2] + q[nve3] * q[nve3];
   const R bsq = q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3];
   const R vn = q[nve1 + D], bn = q[nbm1 + D];
@@ -82,6 +110,9 @@ __device__ __forceinline__ void prim_to_state(const R* q, RState<R>& s) {
   s.cf = sqrt((cts + sqrt(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
   s.vn = vn;
   s.pt = pt;
+  s.vv = vf;
+  s.vw = vb;
+  s.X = q[nst];
 }
 
 // Rotated-frame numerical flux from HLLD (Miyoshi & Kusano 2005, J. Comput. Phys. 208:315)
@@ -102,10 +133,11 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   const R eps = R(1.0e-30);
   const R tiny = R(1.0e-20);
   const R rol = L.rho, eel = L.E, rxl = L.mu, ryl = L.mv, rzl = L.mw;
-  const R vxl = L.mu / L.rho, vyl = L.mv / L.rho, vzl = L.mw / L.rho;
+  // (rho v)/rho of KH/mhd.cpp:173-175 is the reconstructed velocity up to an ulp
+  const R vxl = L.vn, vyl = L.vv, vzl = L.vw;
   const R byl = L.bv, bzl = L.bw, ptl = L.pt, scl = L.sc;
   const R ror = Rt.rho, eer = Rt.E, rxr = Rt.mu, ryr = Rt.mv, rzr = Rt.mw;
-  const R vxr = Rt.mu / Rt.rho, vyr = Rt.mv / Rt.rho, vzr = Rt.mw / Rt.rho;
+  const R vxr = Rt.vn, vyr = Rt.vv, vzr = Rt.vw;
   const R byr = Rt.bv, bzr = Rt.bw, ptr = Rt.pt, scr = Rt.sc;
 
   const R cfmax = fmax(L.cf, Rt.cf);
@@ -116,7 +148,7 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   const R sdr = fmax(tiny, sr - vxr);
   const R rosdl = rol * sdl;
   const R rosdr = ror * sdr;
-  R temp = R(1) / (rosdr - rosdl);
+  R temp = dv(R(1), rosdr - rosdl);
   const R sm = (rosdr * vxr - rosdl * vxl - ptr + ptl) * temp;
   const R sdml = fmin(-tiny, sl - sm);
   const R sdmr = fmax(tiny, sr - sm);
@@ -129,11 +161,11 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   R sign1 = copysign1(fabs(temp_fst) - eps);
   R maxs1 = fmax(R(0), sign1);
   R mins1 = fmin(R(0), sign1);
-  R itf = R(1) / (temp_fst + mins1);
-  const R isdml = R(1) / sdml;
+  R itf = dv(R(1), temp_fst + mins1);
+  const R isdml = dv(R(1), sdml);
   temp = bxs * (sdl - sdml) * itf;
   const R rolst = maxs1 * (rosdl * isdml) - mins1 * rol;
-  const R sclst = rolst / rol * scl;
+  const R sclst = rolst * L.X;  // rolst/rol*scl with scl/rol = X
   const R vxlst = maxs1 * sm - mins1 * vxl;
   const R rxlst = rolst * vxlst;
   const R vylst = maxs1 * (vyl - byl * temp) - mins1 * vyl;
@@ -153,11 +185,11 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   sign1 = copysign1(fabs(temp_fst) - eps);
   maxs1 = fmax(R(0), sign1);
   mins1 = fmin(R(0), sign1);
-  itf = R(1) / (temp_fst + mins1);
-  const R isdmr = R(1) / sdmr;
+  itf = dv(R(1), temp_fst + mins1);
+  const R isdmr = dv(R(1), sdmr);
   temp = bxs * (sdr - sdmr) * itf;
   const R rorst = maxs1 * (rosdr * isdmr) - mins1 * ror;
-  const R scrst = rorst / ror * scr;
+  const R scrst = rorst * Rt.X;
   const R vxrst = maxs1 * sm - mins1 * vxr;
   const R rxrst = rorst * vxrst;
   const R vyrst = maxs1 * (vyr - byr * temp) - mins1 * vyr;
@@ -180,7 +212,7 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   sign1 = copysign1(abbx - eps);
   maxs1 = fmax(R(0), sign1);
   mins1 = -fmin(R(0), sign1);
-  const R invsumro = maxs1 / (sqrtrol + sqrtror);
+  const R invsumro = dv(maxs1, sqrtrol + sqrtror);
 
   temp = invsumro * (sqrtrol * vylst + sqrtror * vyrst + signbx * (byrst - bylst));
   const R vyldst = vylst * mins1 + temp;
@@ -205,8 +237,8 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   const R eerdst = eerst + sqrtror * signbx * (vdbstr - temp) * maxs1;
 
   // flux selection by sign(S_M) (KH/mhd.cpp:381-415)
-  const R slst = (sm - abbx / sqrtrol) * maxs1;
-  const R srst = (sm + abbx / sqrtror) * maxs1;
+  const R slst = (sm - dv(abbx, sqrtrol)) * maxs1;
+  const R srst = (sm + dv(abbx, sqrtror)) * maxs1;
   sign1 = copysign1(sm);
   maxs1 = fmax(R(0), sign1);
   mins1 = -fmin(R(0), sign1);
@@ -234,8 +266,9 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
 // primitive states: HLLD for the MHD part plus the GLM (B_n, psi) flux
 // F(B_n) = (psiL+psiR)/2 - c_h (B_nR-B_nL)/2,  F(psi) = c_h^2 [(B_nL+B_nR)/2 - (psiR-psiL)/(2 c_h)]
 // (scatter and GLM as at KH/mhd.cpp:1225-1240, 1306-1321, 1391-1406).
+// chg: the GLM speed c_h; hchg = 0.5/c_h (hoisted by the caller)
 template <int D, int EFLUX, typename R>
-__device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double chg, double* F) {
+__device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double chg, double hchg, double* F) {
   RState<R> sl, sr;
   prim_to_state<D, EFLUX, R>(ql, sl);
   prim_to_state<D, EFLUX, R>(qr, sr);
@@ -252,7 +285,7 @@ __device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double ch
   // GLM part in FP64 from the reconstructed (B_n, psi)
   const double bnl = double(sl.bu), bnr = double(sr.bu), psl = double(sl.psi), psr = double(sr.psi);
   F[mbm1 + D] = 0.5 * (psl + psr) - 0.5 * chg * (bnr - bnl);
-  F[mbps] = (0.5 * (bnl + bnr) - 0.5 / chg * (psr - psl)) * chg * chg;
+  F[mbps] = (0.5 * (bnl + bnr) - hchg * (psr - psl)) * chg * chg;
 }
 
 // Left state at face f+1/2 and right state at face f-1/2 of cell f, from cells f-1, f, f+1:
@@ -271,14 +304,14 @@ __device__ __forceinline__ void cell_states(const R* qm, const R* q0, const R* q
 // UpdatePrimitvP on one cell, KH/mhd.cpp:1480-1507
 __device__ __forceinline__ void cons_to_prim(const double* u, double gam, double* q) {
   const double rho = u[mden];
-  const double rhoinv = 1.0 / rho;
+  const double rhoinv = dv(1.0, rho);
   q[nden] = rho;
   q[nve1] = u[mrv1] * rhoinv;
   q[nve2] = u[mrv2] * rhoinv;
   q[nve3] = u[mrv3] * rhoinv;
   const double ekin = 0.5 * (u[mrv1] * u[mrv1] + u[mrv2] * u[mrv2] + u[mrv3] * u[mrv3]) * rhoinv;
   const double emag = 0.5 * (u[mbm1] * u[mbm1] + u[mbm2] * u[mbm2] + u[mbm3] * u[mbm3]);
-  const double e = (u[meto] - ekin - emag) / rho;
+  const double e = (u[meto] - ekin - emag) * rhoinv;
   q[nene] = e;
   q[npre] = e * rho * (gam - 1.0);
   q[ncsp] = sqrt(e * gam * (gam - 1.0));
@@ -292,10 +325,10 @@ __device__ __forceinline__ void cons_to_prim(const double* u, double gam, double
 // CFL time-step bound of one cell (KH/mhd.cpp:1542-1553) with the cell widths given
 __device__ __forceinline__ double cell_dt(const double* q, double dx, double dy, double dz) {
   const double ctot =
-      sqrt(q[ncsp] * q[ncsp] + (q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3]) / q[nden]);
-  const double d1 = dx / (fabs(q[nve1]) + ctot);
-  const double d2 = dy / (fabs(q[nve2]) + ctot);
-  const double d3 = dz / (fabs(q[nve3]) + ctot);
+      sqrt(q[ncsp] * q[ncsp] + dv(q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3], q[nden]));
+  const double d1 = dv(dx, fabs(q[nve1]) + ctot);
+  const double d2 = dv(dy, fabs(q[nve2]) + ctot);
+  const double d3 = dv(dz, fabs(q[nve3]) + ctot);
   return fmin(d1, fmin(d2, d3));
 }
 
@@ -303,9 +336,10 @@ __device__ __forceinline__ double cell_dt(const double* q, double dx, double dy,
 __device__ __forceinline__ double cell_ch(const double* q) {
   const double rho = q[nden];
   const double css = q[ncsp] * q[ncsp];
-  const double ca1 = q[nbm1] * q[nbm1] / rho;
-  const double ca2 = q[nbm2] * q[nbm2] / rho;
-  const double ca3 = q[nbm3] * q[nbm3] / rho;
+  const double rinv = dv(1.0, rho);
+  const double ca1 = q[nbm1] * q[nbm1] * rinv;
+  const double ca2 = q[nbm2] * q[nbm2] * rinv;
+  const double ca3 = q[nbm3] * q[nbm3] * rinv;
   const double cts = css + ca1 + ca2 + ca3;
   const double cm1 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca1)) / 2.0);
   const double cm2 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca2)) / 2.0);
