@@ -63,6 +63,43 @@ def cpu_baseline(n: int, steps: int) -> dict:
             "sample": f"{steps} steps of the same {n}^3 box (oracle C++/OpenMP restatement), {wall:.1f} s"}
 
 
+def serving_bench(count: int, seed: int = 20261203) -> dict:
+    """Continuous-batching server (FP32 reconstruction + HLLD, up to 48 live boxes) against the
+    plain FP64 solver serving the same stream one request at a time.
+
+    Requests are shear-layer boxes with n log-uniform in [32, 96] (rounded to 8) and end
+    times uniform in [0.002, 0.012] (a few tens of steps each at CFL factor 0.05; the
+    physical t_end of [1, 6] would need 1e4-1e5 forward-Euler steps per request).  The
+    one-at-a-time capacity is measured first with every request queued at t = 0; the stream
+    is then replayed with Poisson arrivals at 1x and 4x that capacity."""
+    from mhdturbulence_amd import Config
+    from mhdturbulence_amd.serve import Server, poisson_stream
+    base = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
+                energy_flux=1, grid_shift=0)
+    kw = dict(n_range=(32, 96), t_range=(0.002, 0.012), seed=seed)
+    out = {"requests": count, "n_range": [32, 96], "t_end_range": [0.002, 0.012]}
+    cap_stream = poisson_stream(rate=0.0, count=count, **kw)
+    single = Server(Config(**base, precision=0), max_boxes=1, arena_bytes=4 << 30, chunk_steps=10)
+    cap = single.serve(cap_stream)
+    capacity = count / cap["wall_s"]
+    out["fp64_one_at_a_time_capacity"] = {"requests_per_s": capacity, "cell_updates_per_s": cap["cell_updates_per_s"]}
+    for mult in (1.0, 4.0):
+        stream = poisson_stream(rate=mult * capacity, count=count, **kw)
+        one = single.serve(stream)
+        batched = Server(Config(**base, precision=1), max_boxes=48, arena_bytes=32 << 30, chunk_steps=10)
+        bat = batched.serve(stream)
+        batched.close()
+        out[f"rate_{mult:g}x"] = {
+            "arrival_rate_per_s": mult * capacity,
+            "batched_mixed": {k: bat[k] for k in ("cell_updates_per_s", "p50_latency_s", "p99_latency_s", "wall_s")},
+            "fp64_one_at_a_time": {k: one[k] for k in ("cell_updates_per_s", "p50_latency_s", "p99_latency_s",
+                                                      "wall_s")},
+            "throughput_ratio": bat["cell_updates_per_s"] / one["cell_updates_per_s"],
+        }
+    single.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -74,6 +111,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
@@ -102,7 +141,9 @@ def main():
     dominant = max(prof, key=prof.get)
     dom_ms = prof[dominant]
     step_ms = sum(prof.values())
-    # algorithmic bytes per launch: a launch of any step kernel advances the whole box once
+    # algorithmic bytes of one launch of the dominant kernel: the per-cell-update figure (160 B,
+    # read + write of the FP64 conserved state) times the cells that launch processes (every
+    # interior cell); "step" relates the same bytes to the whole step's kernel time
     alg_bytes = BYTES_PER_CELL * n ** 3
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -125,6 +166,9 @@ def main():
         "sim_time": t,
     }
     del box
+    if not args.no_serve:
+        log(f"[bench] serving: {args.serve_requests} requests, batched vs one at a time")
+        out["serving"] = serving_bench(args.serve_requests)
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {n}^3")
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)

@@ -139,6 +139,41 @@ int mhd_profile_kernels(mhd_ctx* ctx, int nsteps, int mode, double* avg_ms, int 
 /* comma-separated names of the launch positions of one iteration of the given mode */
 const char* mhd_kernel_names(int mode);
 
+/* ---- batches: many boxes advanced together ----
+ * No counterpart in the reference, which runs one box per MPI job.  A batch holds up to
+ * max_boxes boxes of different sizes and initial states that share base's physics
+ * (gam, cfl, alphabp, bc, domain, energy_flux, grid_shift, precision, device).  Every
+ * step issues one launch per fused kernel for all live boxes; each box keeps its own dt /
+ * c_h reduction, time and stop flag (it stops after the step that passes its tmax).  Box
+ * storage comes from one arena of arena_bytes allocated at creation, so adding and
+ * removing boxes allocates nothing on the device; MHD_ERR_NOMEM means "does not fit now". */
+typedef struct mhd_batch mhd_batch;
+int mhd_batch_create(const mhd_config* base, int max_boxes, uint64_t arena_bytes, mhd_batch** out);
+int mhd_batch_destroy(mhd_batch* batch);
+/* arena bytes one box of this size takes (the admission test) */
+int mhd_batch_box_bytes(const mhd_batch* batch, int nx, int ny, int nz, uint64_t* bytes);
+int mhd_batch_free_bytes(const mhd_batch* batch, uint64_t* largest_free, uint64_t* total_free);
+/* tile shape for boxes of size nx*ny*nz added later (sw: 16/32/64 x-strip, ly/lz: march chunks) */
+int mhd_batch_set_tiling(mhd_batch* batch, int nx, int ny, int nz, int sw, int ly, int lz);
+/* grid (GenerateGrid) and, if ic != NULL, the shear-layer state on the device; tmax <= 0: no stop */
+int mhd_batch_add_box(mhd_batch* batch, int nx, int ny, int nz, const mhd_ic_params* ic, double tmax, int* box_id);
+/* replace a box's state by host arrays (reference layout) and restart its clock at 0 */
+int mhd_batch_upload_state(mhd_batch* batch, int box_id, const double* P, const double* U);
+int mhd_batch_remove_box(mhd_batch* batch, int box_id);
+/* nsteps loop iterations of every live box; returns when they are done */
+int mhd_batch_step(mhd_batch* batch, int nsteps);
+int mhd_batch_last_step_ms(const mhd_batch* batch, double* ms);
+int mhd_batch_box_state(mhd_batch* batch, int box_id, double* time, int* steps_done, int* stopped, int* status);
+/* ids of live boxes that have stopped (n may exceed cap; ids holds the first cap) */
+int mhd_batch_poll(mhd_batch* batch, int* ids, int cap, int* n);
+/* primitive recovery of the box's current state; diag4 (may be NULL) as mhd_diagnostics,
+ * P_host (may be NULL) the primitive array in the reference layout */
+int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_host);
+
+/* tile shape of a single box's fused kernels (sw: 16/32/64, 1 <= ly <= ny, 1 <= lz <= nz) */
+int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
+int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
+
 #ifdef __cplusplus
 }
 #endif

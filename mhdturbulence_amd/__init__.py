@@ -3,6 +3,6 @@
 The compute path is libmhd_amd.so (hand-written HIP kernels for gfx950 behind the C ABI
 in include/mhd_amd.h); this package binds it and mirrors the reference interface.
 """
-from .solver import Box, Config, PERIODIC, REFLECTION, OUTFLOW  # noqa: F401
+from .solver import Batch, Box, Config, PERIODIC, REFLECTION, OUTFLOW  # noqa: F401
 
-__all__ = ["Box", "Config", "PERIODIC", "REFLECTION", "OUTFLOW"]
+__all__ = ["Batch", "Box", "Config", "PERIODIC", "REFLECTION", "OUTFLOW"]

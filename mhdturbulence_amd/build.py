@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["mhd_kernels.hip", "mhd_fused.hip", "mhd_capi.hip"]  # missing ones are skipped
+SOURCES = ["mhd_kernels.hip", "mhd_fused.hip", "mhd_capi.hip", "mhd_batch.hip"]
 
 
 def _run(cmd):

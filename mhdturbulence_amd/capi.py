@@ -20,6 +20,10 @@ EXPORTS = (
     "mhd_update_conserv", "mhd_damp_psi", "mhd_update_primitive", "mhd_run", "mhd_generate_problem_kh",
     "mhd_generate_problem_shear", "mhd_diagnostics", "mhd_status", "mhd_last_run_timing",
     "mhd_profile_kernels", "mhd_kernel_names",
+    "mhd_batch_create", "mhd_batch_destroy", "mhd_batch_box_bytes", "mhd_batch_free_bytes", "mhd_batch_set_tiling",
+    "mhd_batch_add_box", "mhd_batch_upload_state", "mhd_batch_remove_box", "mhd_batch_step",
+    "mhd_batch_last_step_ms", "mhd_batch_box_state", "mhd_batch_poll", "mhd_batch_finish_box",
+    "mhd_set_tiling", "mhd_get_tiling",
 )
 
 
@@ -89,6 +93,21 @@ def load(path: str = LIB_PATH):
         "mhd_last_run_timing": (C.c_int, [vp, dp, ip]),
         "mhd_profile_kernels": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, ip]),
         "mhd_kernel_names": (C.c_char_p, [C.c_int]),
+        "mhd_batch_create": (C.c_int, [C.POINTER(MhdConfig), C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "mhd_batch_destroy": (C.c_int, [vp]),
+        "mhd_batch_box_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
+        "mhd_batch_free_bytes": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "mhd_batch_set_tiling": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "mhd_batch_add_box": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(MhdIcParams), C.c_double, ip]),
+        "mhd_batch_upload_state": (C.c_int, [vp, C.c_int, dp, dp]),
+        "mhd_batch_remove_box": (C.c_int, [vp, C.c_int]),
+        "mhd_batch_step": (C.c_int, [vp, C.c_int]),
+        "mhd_batch_last_step_ms": (C.c_int, [vp, dp]),
+        "mhd_batch_box_state": (C.c_int, [vp, C.c_int, dp, ip, ip, ip]),
+        "mhd_batch_poll": (C.c_int, [vp, ip, C.c_int, ip]),
+        "mhd_batch_finish_box": (C.c_int, [vp, C.c_int, dp, dp]),
+        "mhd_set_tiling": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mhd_get_tiling": (C.c_int, [vp, ip, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

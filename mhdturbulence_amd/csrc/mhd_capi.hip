@@ -17,14 +17,22 @@
 using mhd::Geom;
 using mhd::StepScalars;
 
-namespace {
+namespace mhd_capi_detail {
 
 thread_local std::string g_last_error;
 
+// records msg as this thread's last error and returns code (shared by every C-ABI source)
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+
+}  // namespace mhd_capi_detail
+
+namespace {
+
+using mhd_capi_detail::fail;
+using mhd_capi_detail::g_last_error;
 
 #define HIPCHK(expr)                                                                                 \
   do {                                                                                               \
@@ -550,6 +558,42 @@ int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int ca
 }
 
 const char* mhd_kernel_names(int mode) { return kernel_names(mode); }
+
+int mhd_set_tiling(mhd_ctx* c, int sw, int ly, int lz) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  if ((sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > c->g.ny || lz > c->g.nz)
+    return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= nz");
+  CHKRET(set_dev(c));
+  CHKRET(ensure_fused(c));
+  const int ns = (c->g.nx + sw - 1) / sw;
+  mhd::FusedArgs a = c->fa;
+  a.sw = sw;
+  a.ns = ns;
+  a.ly = ly;
+  a.lz = lz;
+  if (mhd::fused_edge_elems(a) > mhd::fused_edge_elems(c->fa)) {
+    // the strip-edge buffer grows with the number of strips
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->bytes -= mhd::fused_edge_elems(c->fa) * sizeof(double);
+    (void)hipFree(c->edge);
+    c->edge = nullptr;
+    CHKRET(dalloc(c, &c->edge, mhd::fused_edge_elems(a)));
+    a.edge = c->edge;
+  }
+  c->fa = a;
+  drop_graph(c);
+  return MHD_OK;
+}
+
+int mhd_get_tiling(mhd_ctx* c, int* sw, int* ly, int* lz) {
+  if (!c || !sw || !ly || !lz) return fail(MHD_ERR_ARG, "null argument");
+  CHKRET(set_dev(c));
+  CHKRET(ensure_fused(c));
+  *sw = c->fa.sw;
+  *ly = c->fa.ly;
+  *lz = c->fa.lz;
+  return MHD_OK;
+}
 
 int mhd_generate_problem_kh(mhd_ctx* c, double vy_amp, double rrv) {
   CHKRET(need_grid(c));

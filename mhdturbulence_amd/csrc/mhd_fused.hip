@@ -107,14 +107,13 @@ __device__ __forceinline__ size_t edge_index(const FusedArgs& a, int m, int k, i
 // x faces at the left edge of each strip and the right edge of the last one:
 // face index f = NGH + min(e*sw, nx), e in [0, ns]; one thread per (e, j, k).
 template <int EFLUX, typename R>
-__global__ __launch_bounds__(256) void k_fused_edge(FusedArgs a) {
+__device__ __forceinline__ void edge_body(const FusedArgs& a, long long t) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
   const int iter = sc->iter;
   const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
   const double chg = bits2d(sc->red[iter % 3][1]);
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int ne = a.ns + 1;
   if (t >= (long long)ne * g.ny * g.nz) return;
   const int e = (int)(t % ne);
@@ -134,9 +133,15 @@ __global__ __launch_bounds__(256) void k_fused_edge(FusedArgs a) {
   for (int m = 0; m < NCONS; ++m) a.edge[edge_index(a, m, k, j, e)] = F[m];
 }
 
+template <int EFLUX, typename R>
+__global__ __launch_bounds__(256) void k_fused_edge(FusedArgs a) {
+  edge_body<EFLUX, R>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------- x + y divergence
-template <int SW, int EFLUX, typename R>
-__global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
+// one wavefront: x-strip s of sw cells (64/sw strips stacked in k), rows [j0, j1) of chunk yc
+template <int EFLUX, typename R>
+__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -145,10 +150,9 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
   const double chg = bits2d(sc->red[iter % 3][1]);
   const double hchg = 0.5 / chg;
-  constexpr int SPW = 64 / SW;
-  const int lane = threadIdx.x & 63;
+  const int SW = a.sw;
+  const int SPW = 64 / SW;
   const int sub = lane / SW, l = lane - sub * SW;
-  const int w = blockIdx.x * WAVES + (threadIdx.x >> 6);
   const int nkg = (g.nz + SPW - 1) / SPW;
   const int nyc = (g.ny + a.ly - 1) / a.ly;
   const int s = w % a.ns;
@@ -252,9 +256,15 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   }
 }
 
-// ---------------------------------------------------------------- z + update + recovery + reduction
 template <int EFLUX, typename R>
-__global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
+__global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
+  xy_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// ---------------------------------------------------------------- z + update + recovery + reduction
+// one wavefront: 64 x-cells at row j, planes [k0, k1) of chunk zc
+template <int EFLUX, typename R>
+__device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -266,8 +276,6 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
   const double dt = a.cfl * bits2d(rd[0]);
   const double chg = bits2d(rd[1]);
   const double hchg = 0.5 / chg;
-  const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * WAVES + (threadIdx.x >> 6);
   const int nxs = (g.nx + 63) / 64;
   const int nzc = (g.nz + a.lz - 1) / a.lz;
   const int s = w % nxs;
@@ -373,38 +381,103 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
   }
 }
 
+template <int EFLUX, typename R>
+__global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
+  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// ---------------------------------------------------------------- batched launches
+// Many boxes in one launch: tab[block] = (box, block index within that box).  Every box's
+// block count is a whole number of blocks, so a block never straddles two boxes and the
+// box descriptor read is uniform over the block.
+template <int EFLUX, typename R>
+__global__ __launch_bounds__(256) void k_batch_edge(const FusedArgs* __restrict__ boxes, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
+  edge_body<EFLUX, R>(a, (long long)e.y * blockDim.x + threadIdx.x);
+}
+
+template <int EFLUX, typename R>
+__global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
+                                                                     const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
+  xy_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+template <int EFLUX, typename R>
+__global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const FusedArgs* __restrict__ boxes,
+                                                                    const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
+  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// step end of every live box (k_step_end with the fused ring offset), one thread per box
+__global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int* __restrict__ live, int nlive,
+                                 double* __restrict__ dt_last) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlive) return;
+  const int b = live[t];
+  const FusedArgs& a = boxes[b];
+  StepScalars* sc = a.sc;
+  if (sc->stopped) return;
+  const double dt = a.cfl * bits2d(sc->red[sc->iter % 3][0]);
+  sc->time += dt;
+  sc->steps_done += 1;
+  sc->iter += 1;
+  unsigned long long* nxt = sc->red[(sc->iter + 1) % 3];
+  nxt[0] = DT_INIT_BITS;
+  nxt[1] = CH_INIT_BITS;
+  if (dt_last) dt_last[b] = dt;
+  if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
+}
+
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 template <int EFLUX, typename R>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
-  const int spw = 64 / a.sw;
-  const long long waves = (long long)a.ns * ((a.g.nz + spw - 1) / spw) * ((a.g.ny + a.ly - 1) / a.ly);
-  const dim3 grid(cdiv(waves, WAVES)), blk(64 * WAVES);
-  if (a.sw == 64) hipLaunchKernelGGL((k_fused_xy<64, EFLUX, R>), grid, blk, 0, s, a);
-  else if (a.sw == 32) hipLaunchKernelGGL((k_fused_xy<32, EFLUX, R>), grid, blk, 0, s, a);
-  else hipLaunchKernelGGL((k_fused_xy<16, EFLUX, R>), grid, blk, 0, s, a);
+  hipLaunchKernelGGL((k_fused_xy<EFLUX, R>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), 0, s, a);
 }
 
 template <int EFLUX, typename R>
 void launch_z_t(const FusedArgs& a, hipStream_t s) {
-  const long long waves = (long long)((a.g.nx + 63) / 64) * a.g.ny * ((a.g.nz + a.lz - 1) / a.lz);
-  hipLaunchKernelGGL((k_fused_z<EFLUX, R>), dim3(cdiv(waves, WAVES)), dim3(64 * WAVES), 0, s, a);
+  hipLaunchKernelGGL((k_fused_z<EFLUX, R>), dim3(fused_blocks_z(a)), dim3(64 * WAVES), 0, s, a);
 }
 
 template <int EFLUX, typename R>
 void launch_edge_t(const FusedArgs& a, hipStream_t s) {
-  const long long n = (long long)(a.ns + 1) * a.g.ny * a.g.nz;
-  hipLaunchKernelGGL((k_fused_edge<EFLUX, R>), dim3(cdiv(n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_fused_edge<EFLUX, R>), dim3(fused_blocks_edge(a)), dim3(256), 0, s, a);
 }
 
-#define MHD_DISPATCH(fn, a, s)                                   \
-  do {                                                           \
-    if ((a).eflux == 0) {                                        \
-      if ((a).mixed) fn<0, float>(a, s); else fn<0, double>(a, s); \
-    } else {                                                     \
-      if ((a).mixed) fn<1, float>(a, s); else fn<1, double>(a, s); \
-    }                                                            \
+struct BatchLaunch {
+  const FusedArgs* boxes;
+  const int2* tab;
+  int nblocks;
+};
+
+template <int EFLUX, typename R>
+void launch_bedge_t(const BatchLaunch& b, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_edge<EFLUX, R>), dim3(b.nblocks), dim3(256), 0, s, b.boxes, b.tab);
+}
+template <int EFLUX, typename R>
+void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), 0, s, b.boxes, b.tab);
+}
+template <int EFLUX, typename R>
+void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_z<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), 0, s, b.boxes, b.tab);
+}
+
+#define MHD_DISPATCH2(fn, eflux, mixed, a, s)                 \
+  do {                                                         \
+    if ((eflux) == 0) {                                        \
+      if (mixed) fn<0, float>(a, s); else fn<0, double>(a, s); \
+    } else {                                                   \
+      if (mixed) fn<1, float>(a, s); else fn<1, double>(a, s); \
+    }                                                          \
   } while (0)
+#define MHD_DISPATCH(fn, a, s) MHD_DISPATCH2(fn, (a).eflux, (a).mixed, a, s)
 
 }  // namespace
 
@@ -422,6 +495,33 @@ void fused_configure(FusedArgs& a) {
 }
 
 size_t fused_edge_elems(const FusedArgs& a) { return (size_t)NCONS * a.g.nz * a.g.ny * (a.ns + 1); }
+
+int fused_blocks_edge(const FusedArgs& a) { return (int)cdiv((long long)(a.ns + 1) * a.g.ny * a.g.nz, 256); }
+
+int fused_blocks_xy(const FusedArgs& a) {
+  const int spw = 64 / a.sw;
+  const long long waves = (long long)a.ns * ((a.g.nz + spw - 1) / spw) * ((a.g.ny + a.ly - 1) / a.ly);
+  return (int)cdiv(waves, WAVES);
+}
+
+int fused_blocks_z(const FusedArgs& a) {
+  const long long waves = (long long)((a.g.nx + 63) / 64) * a.g.ny * ((a.g.nz + a.lz - 1) / a.lz);
+  return (int)cdiv(waves, WAVES);
+}
+
+void launch_batch_edge(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bedge_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
+}
+void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
+}
+void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
+}
+void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s) {
+  if (nlive > 0)
+    hipLaunchKernelGGL(k_batch_step_end, dim3(cdiv(nlive, 256)), dim3(256), 0, s, boxes, live, nlive, dt_last);
+}
 
 void launch_fused_edge(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_edge_t, a, s); }
 void launch_fused_xy(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_xy_t, a, s); }

@@ -39,8 +39,21 @@ struct FusedArgs {
 void fused_configure(FusedArgs& a);
 size_t fused_edge_elems(const FusedArgs& a);
 
+// blocks of 256 threads each kernel needs for one box
+int fused_blocks_edge(const FusedArgs& a);
+int fused_blocks_xy(const FusedArgs& a);
+int fused_blocks_z(const FusedArgs& a);
+
 void launch_fused_edge(const FusedArgs& a, hipStream_t s);
 void launch_fused_xy(const FusedArgs& a, hipStream_t s);
 void launch_fused_z(const FusedArgs& a, hipStream_t s);
+
+// Batched launches: boxes[] are device-resident descriptors, tab[block] = (box, block of
+// that box); nblocks = tab length.  All boxes of a batch share eflux and precision.
+void launch_batch_edge(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
+void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
+void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
+// time += dt etc. for each listed box (live[] holds box indices); dt_last[box] gets its dt
+void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s);
 
 }  // namespace mhd

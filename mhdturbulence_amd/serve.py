@@ -1,0 +1,152 @@
+"""Request server: continuous batching of simulation requests on one MI355X.
+
+Requests (a shear-layer box of n^3 cells run to time t_end) arrive over time.  The server
+
+* admits a pending request as soon as a box slot and an arena region of its HBM
+  footprint are free (FIFO, no reordering);
+* keeps every admitted box resident in HBM and advances all live boxes together, one
+  batched launch per kernel per step (mhd_batch_*);
+* retires a box when it passes its t_end (each box stops on its own clock), returns its
+  t-prof diagnostics (volume, mix, sqrt(<vy^2>), max|divB|) and frees its region for the
+  next request.
+
+`serve()` replays a request stream against the wall clock and reports per-request latency
+(completion - arrival), p50 / p99 and aggregate cell-updates/s.  With max_boxes=1 the same
+loop is the one-request-at-a-time baseline.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .solver import Batch, Config
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    t_end: float
+    ic: dict
+    arrival: float = 0.0
+
+
+@dataclass
+class Result:
+    rid: int
+    n: int
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    sim_time: float
+    diagnostics: list = field(default_factory=list)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def splitmix64(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def request_params(rid: int, seed: int = 20261203) -> dict:
+    """per-request shear-layer parameters: a in [0.02,0.1], Bx in [0,0.2], amplitude in
+    [1e-3,1e-2], p in [0.5,5], drawn from splitmix64(seed + rid)"""
+    s = seed + rid
+    u = []
+    for _ in range(4):
+        s = splitmix64(s)
+        u.append((s >> 11) * 2.0 ** -53)
+    return dict(a=0.02 + 0.08 * u[0], bx0=0.2 * u[1], amp=1e-3 + 9e-3 * u[2], p0=0.5 + 4.5 * u[3],
+                seed=(seed + rid) & 0xFFFFFFFFFFFFFFFF)
+
+
+def poisson_stream(rate: float, count: int, seed: int = 20261203, n_range=(32, 256), t_range=(1.0, 6.0),
+                   multiple: int = 8) -> list[Request]:
+    """Poisson arrivals at `rate` per second; n log-uniform in n_range rounded to a multiple
+    of `multiple`; t_end uniform in t_range."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
+        n = int(max(n_range[0], min(n_range[1], round(n / multiple) * multiple)))
+        out.append(Request(rid, n, float(rng.uniform(*t_range)), request_params(rid, seed), t))
+    return out
+
+
+class Server:
+    def __init__(self, cfg: Config, max_boxes: int = 64, arena_bytes: int = 64 << 30, chunk_steps: int = 10,
+                 tilings: dict | None = None):
+        self.batch = Batch(cfg, max_boxes, arena_bytes)
+        self.chunk = chunk_steps
+        for (nx, ny, nz), (sw, ly, lz) in (tilings or {}).items():
+            self.batch.set_tiling(nx, ny, nz, sw, ly, lz)
+
+    def close(self):
+        self.batch.close()
+
+    def footprint(self, n: int) -> int:
+        return self.batch.box_bytes(n, n, n)
+
+    def serve(self, requests: list[Request], clock=time.perf_counter, log=None) -> dict:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        t0 = clock()
+        nxt = 0
+        pending: list[Request] = []
+        live: dict[int, tuple[Request, float]] = {}
+        results: list[Result] = []
+        cell_updates = 0
+        gpu_ms = 0.0
+        while len(results) < len(reqs):
+            now = clock() - t0
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                pending.append(reqs[nxt])
+                nxt += 1
+            # admission in arrival order while a slot and a region of the footprint are free
+            while pending:
+                r = pending[0]
+                bid = self.batch.try_add_box(r.n, r.n, r.n, tmax=r.t_end, ic=r.ic)
+                if bid is None:
+                    if not live:
+                        raise RuntimeError(f"request {r.rid} (n={r.n}) does not fit the arena even alone")
+                    break
+                live[bid] = (r, clock() - t0)
+                pending.pop(0)
+            if not live:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, reqs[nxt].arrival - (clock() - t0)))
+                continue
+            gpu_ms += self.batch.step(self.chunk)
+            for bid in self.batch.poll():
+                r, adm = live.pop(bid)
+                st = self.batch.box_state(bid)
+                diag, _ = self.batch.finish_box(bid)
+                self.batch.remove_box(bid)
+                cell_updates += st["steps"] * r.n ** 3
+                res = Result(r.rid, r.n, r.arrival, adm, clock() - t0, st["steps"], st["time"], diag.tolist())
+                results.append(res)
+                if log:
+                    log(f"done rid={r.rid} n={r.n} steps={st['steps']} latency={res.latency:.3f}s")
+        wall = clock() - t0
+        lat = np.array([r.latency for r in results])
+        return {
+            "requests": len(results),
+            "wall_s": wall,
+            "gpu_s": gpu_ms / 1e3,
+            "cell_updates": cell_updates,
+            "cell_updates_per_s": cell_updates / wall,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "results": results,
+        }

@@ -184,6 +184,15 @@ class Box:
         names = self.L.mhd_kernel_names(mode).decode().split(",")
         return dict(zip(names, ms[:n.value].tolist()))
 
+    def set_tiling(self, sw: int, ly: int, lz: int):
+        """x-strip width (16/32/64) and y / z march chunk lengths of the fused kernels"""
+        capi.check(self.L.mhd_set_tiling(self.h, sw, ly, lz))
+
+    def get_tiling(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        capi.check(self.L.mhd_get_tiling(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def last_run_timing(self) -> tuple[float, int]:
         ms, n = C.c_double(), C.c_int()
         capi.check(self.L.mhd_last_run_timing(self.h, C.byref(ms), C.byref(n)))
@@ -212,3 +221,100 @@ class Box:
     def interior(self, arr: np.ndarray) -> np.ndarray:
         """(nv, nz, ny, nx) copy of the interior cells of a (ktot, jtot, nv, itot) array."""
         return np.ascontiguousarray(arr[2:self.ktot - 3, 2:self.jtot - 3, :, 2:self.itot - 3].transpose(2, 0, 1, 3))
+
+
+class Batch:
+    """Many boxes advanced together (mhd_batch_* of include/mhd_amd.h).
+
+    All boxes share the physics of `cfg` (its nx/ny/nz are ignored); sizes and initial
+    states are per box.  Storage is one HBM arena of `arena_bytes`."""
+
+    def __init__(self, cfg: Config, max_boxes: int, arena_bytes: int):
+        self.cfg = cfg
+        self.L = capi.load()
+        self._c = cfg.to_c()
+        h = C.c_void_p()
+        capi.check(self.L.mhd_batch_create(C.byref(self._c), max_boxes, arena_bytes, C.byref(h)))
+        self.h = h
+        self.max_boxes = max_boxes
+        self.shapes: dict[int, tuple[int, int, int]] = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def box_bytes(self, nx: int, ny: int, nz: int) -> int:
+        b = C.c_uint64()
+        capi.check(self.L.mhd_batch_box_bytes(self.h, nx, ny, nz, C.byref(b)))
+        return b.value
+
+    def free_bytes(self) -> tuple[int, int]:
+        a, t = C.c_uint64(), C.c_uint64()
+        capi.check(self.L.mhd_batch_free_bytes(self.h, C.byref(a), C.byref(t)))
+        return a.value, t.value
+
+    def set_tiling(self, nx, ny, nz, sw, ly, lz):
+        capi.check(self.L.mhd_batch_set_tiling(self.h, nx, ny, nz, sw, ly, lz))
+
+    def add_box(self, nx: int, ny: int, nz: int, tmax: float = 0.0, ic: dict | None = None,
+                generate: bool = True) -> int:
+        """returns the box id; raises capi.MhdError (MHD_ERR_NOMEM) when it does not fit"""
+        bid = C.c_int()
+        p = None
+        if generate:
+            d = dict(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1, seed=20261203)
+            d.update(ic or {})
+            p = capi.MhdIcParams(d["rho0"], d["p0"], d["v0"], d["a"], d["bx0"], d["amp"], d["sigma"], d["seed"])
+        capi.check(self.L.mhd_batch_add_box(self.h, nx, ny, nz, C.byref(p) if p is not None else None, tmax,
+                                            C.byref(bid)))
+        self.shapes[bid.value] = (nx, ny, nz)
+        return bid.value
+
+    def try_add_box(self, *a, **kw) -> int | None:
+        try:
+            return self.add_box(*a, **kw)
+        except capi.MhdError as e:
+            if "(-3)" in str(e):
+                return None
+            raise
+
+    def upload_state(self, bid: int, P: np.ndarray, U: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        capi.check(self.L.mhd_batch_upload_state(self.h, bid, _dp(P), _dp(U)))
+
+    def remove_box(self, bid: int):
+        capi.check(self.L.mhd_batch_remove_box(self.h, bid))
+        self.shapes.pop(bid, None)
+
+    def step(self, nsteps: int) -> float:
+        """advances every live box nsteps iterations; returns the GPU ms they took"""
+        capi.check(self.L.mhd_batch_step(self.h, nsteps))
+        ms = C.c_double()
+        capi.check(self.L.mhd_batch_last_step_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def box_state(self, bid: int) -> dict:
+        t, n, s, st = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        capi.check(self.L.mhd_batch_box_state(self.h, bid, C.byref(t), C.byref(n), C.byref(s), C.byref(st)))
+        return {"time": t.value, "steps": n.value, "stopped": bool(s.value), "status": st.value}
+
+    def poll(self) -> list[int]:
+        ids = (C.c_int * self.max_boxes)()
+        n = C.c_int()
+        capi.check(self.L.mhd_batch_poll(self.h, ids, self.max_boxes, C.byref(n)))
+        return list(ids[:min(n.value, self.max_boxes)])
+
+    def finish_box(self, bid: int, want_prim: bool = False):
+        """(diagnostics[4], primitive array or None) of the box's current state"""
+        d = np.zeros(4)
+        P = None
+        if want_prim:
+            nx, ny, nz = self.shapes[bid]
+            P = np.zeros((nz + 5, ny + 5, NPRIM, nx + 5))
+        capi.check(self.L.mhd_batch_finish_box(self.h, bid, _dp(d), _dp(P) if P is not None else None))
+        return d, P
