@@ -348,6 +348,10 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
     *dst[q] = reinterpret_cast<double*>(base + o);
     o += sz[q];
   }
+  // staging for host-layout copies spans U1 and acc
+  if (reinterpret_cast<char*>(x.acc) != reinterpret_cast<char*>(x.U1) + sz[2] ||
+      2 * sz[2] < (uint64_t)mhd::NPRIM * x.g.itot * x.g.jtot * x.g.ktot * 8)
+    return fail(MHD_ERR_STATE, "box layout: staging area too small");
   // the whole region starts from zero so that the never-read padding is finite
   BHIP(hipMemsetAsync(base, 0, total, b->stream));
   mhd::launch_generate_grid(x.g, b->base.xmin, b->base.xmax, b->base.grid_shift, x.grid[0], x.grid[1], x.grid[2],
@@ -387,8 +391,9 @@ int mhd_batch_upload_state(mhd_batch* b, int id, const double* P, const double* 
   BHIP(hipSetDevice(b->dev));
   BatchBox& x = b->boxes[id];
   const size_t cells = (size_t)x.g.itot * x.g.jtot * x.g.ktot;
-  // acc serves as staging for the host-layout copy (12 variables fit in acc + U1)
-  double* stage = x.acc;
+  // U1 and acc are adjacent in the box region (20 variables of padded rows), which holds
+  // the 12 unpadded host-layout variables; neither carries state before the restart
+  double* stage = x.U1;
   BHIP(hipMemcpyAsync(stage, P, mhd::NPRIM * cells * 8, hipMemcpyHostToDevice, b->stream));
   mhd::launch_from_host_layout(x.g, mhd::NPRIM, stage, x.P, b->stream);
   BHIP(hipMemcpyAsync(stage, U, mhd::NCONS * cells * 8, hipMemcpyHostToDevice, b->stream));
@@ -503,8 +508,10 @@ int mhd_batch_finish_box(mhd_batch* b, int id, double* diag4, double* P_host) {
   }
   if (P_host) {
     const size_t cells = (size_t)x.g.itot * x.g.jtot * x.g.ktot;
-    mhd::launch_to_host_layout(x.g, mhd::NPRIM, x.P, x.acc, b->stream);
-    BHIP(hipMemcpyAsync(P_host, x.acc, mhd::NPRIM * cells * 8, hipMemcpyDeviceToHost, b->stream));
+    // U1 + acc (adjacent, 20 padded variables) stage the 12 host-layout variables; the state
+    // is in U0 at this point
+    mhd::launch_to_host_layout(x.g, mhd::NPRIM, x.P, x.U1, b->stream);
+    BHIP(hipMemcpyAsync(P_host, x.U1, mhd::NPRIM * cells * 8, hipMemcpyDeviceToHost, b->stream));
   }
   BHIP(hipGetLastError());
   BHIP(hipStreamSynchronize(b->stream));
