@@ -31,6 +31,14 @@ sys.path.insert(0, HERE)
 BYTES_PER_CELL = 2 * 10 * 8      # read + write the FP64 conserved state (NV = 10)
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on MI355X, each
+# counter scaled by what it reads for a device copy of known size (fetch x2.0, write x1.0);
+# summary and commands in profiles/fused_fp64_256cube_pmc.md.  Keyed by
+# (n, mode, precision, kernel); a workload without a measurement reports null.
+MEASURED_TRAFFIC = {
+    (256, 0, 0, "fused_xy"): 995389.90625 * 1024 * 2.0 + 1310720.0 * 1024 * 1.0,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -147,7 +155,8 @@ def main():
     alg_bytes = BYTES_PER_CELL * n ** 3
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get((n, args.mode, args.precision, dominant)), "kernel": dominant,
                 "kernel_ms": round(dom_ms, 4),
                 "step": {"achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2),
                          "frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}

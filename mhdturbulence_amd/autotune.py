@@ -1,0 +1,99 @@
+"""Tile-shape autotuner for the fused step kernels.
+
+For a box shape it times the fused kernels (HIP events, mhd_profile_kernels) for each
+candidate x-strip width (16/32/64 lanes) and y / z march chunk lengths, keeps the
+fastest, and caches the result in a JSON file keyed by shape, precision and energy-flux
+form.  Batches and servers apply the cached tiling per box shape.
+
+    python -m mhdturbulence_amd.autotune 256 256 256 [--precision 0|1]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+
+from .solver import Box, Config
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiling_cache.json")
+
+
+def candidates(nx: int, ny: int, nz: int):
+    sws = [sw for sw in (64, 32, 16) if sw <= max(16, nx)]
+    lys = sorted({min(ny, v) for v in (8, 16, 32, 64)})
+    lzs = sorted({min(nz, v) for v in (8, 16, 32, 64)})
+    return list(itertools.product(sws, lys, lzs))
+
+
+def key(nx, ny, nz, precision, eflux) -> str:
+    return f"{nx}x{ny}x{nz}/p{precision}/e{eflux}"
+
+
+def tune(nx: int, ny: int, nz: int, cfg: Config | None = None, steps: int = 4, cands=None, log=None) -> dict:
+    base = cfg or Config(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0,
+                         energy_flux=1, grid_shift=0)
+    c = Config(**{**base.__dict__, "nx": nx, "ny": ny, "nz": nz})
+    box = Box(c)
+    box.generate_grid()
+    box.generate_problem_shear()
+    box.run(2, mode=0)
+    results = []
+    for sw, ly, lz in cands or candidates(nx, ny, nz):
+        box.set_tiling(sw, ly, lz)
+        box.profile_kernels(1, mode=0)  # first launch of this tiling outside the timing
+        prof = box.profile_kernels(steps, mode=0)
+        ms = prof["fused_edge_x"] + prof["fused_xy"] + prof["fused_z_update"]
+        results.append({"sw": sw, "ly": ly, "lz": lz, "ms": ms})
+        if log:
+            log(f"sw={sw} ly={ly} lz={lz}: {ms:.4f} ms")
+    box.close()
+    best = min(results, key=lambda r: r["ms"])
+    return {"shape": [nx, ny, nz], "precision": c.precision, "energy_flux": c.energy_flux, "best": best,
+            "all": results}
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> dict:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_cache(cache: dict, path: str = DEFAULT_CACHE):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tilings_for(shapes, precision: int = 0, eflux: int = 1, path: str = DEFAULT_CACHE) -> dict:
+    """{(nx,ny,nz): (sw, ly, lz)} for the cached shapes among `shapes`"""
+    cache = load_cache(path)
+    out = {}
+    for s in shapes:
+        e = cache.get(key(*s, precision, eflux))
+        if e:
+            b = e["best"]
+            out[tuple(s)] = (b["sw"], b["ly"], b["lz"])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("shape", type=int, nargs=3)
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--cache", default=DEFAULT_CACHE)
+    a = ap.parse_args()
+    cfg = Config(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, energy_flux=1,
+                 grid_shift=0, precision=a.precision)
+    r = tune(*a.shape, cfg=cfg, steps=a.steps, log=print)
+    cache = load_cache(a.cache)
+    cache[key(*a.shape, a.precision, 1)] = r
+    save_cache(cache, a.cache)
+    print(json.dumps(r["best"]))
+
+
+if __name__ == "__main__":
+    main()

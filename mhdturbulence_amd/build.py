@@ -1,6 +1,8 @@
 """Build the HIP library (libmhd_amd.so) and the C++ drop-in host library in-tree.
 
     python -m mhdturbulence_amd.build          # compile for gfx950
+    python -m mhdturbulence_amd.build -D MHD_PREFETCH=0 --out /tmp/lib_nopf.so   # a variant;
+        # load it with MHD_AMD_LIB=/tmp/lib_nopf.so
 
 Each .hip source is compiled to an object with hipcc for gfx950, then linked into
 mhdturbulence_amd/libmhd_amd.so.  The drop-in host layer (mhdturbulence_amd/host/)
@@ -95,5 +97,19 @@ def build(verbose: bool = True) -> None:
     build_host(verbose)
 
 
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="build libmhd_amd.so (and the drop-in host layer) for gfx950")
+    ap.add_argument("-q", "--quiet", action="store_true")
+    ap.add_argument("-D", "--define", action="append", default=[],
+                    help="compile switch NAME=VALUE for a build variant (MHD_FAST_DIV, MHD_PREFETCH, MHD_F64_MINWAVES)")
+    ap.add_argument("--out", default=LIB, help="library path; a variant elsewhere skips the host layer")
+    a = ap.parse_args(argv)
+    if a.define or a.out != LIB:
+        build_hip(not a.quiet, defines=a.define, out=os.path.abspath(a.out))
+    else:
+        build(not a.quiet)
+
+
 if __name__ == "__main__":
-    build(verbose="-q" not in sys.argv)
+    main()

@@ -51,11 +51,13 @@ class MhdError(RuntimeError):
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Load libmhd_amd.so and declare every signature.  Raises if it is not built."""
+def load(path: str | None = None):
+    """Load libmhd_amd.so (or the build variant named by $MHD_AMD_LIB) and declare every
+    signature.  Raises if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("MHD_AMD_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise MhdError(f"{path} is not built; run `python -m mhdturbulence_amd.build` (HIP path is mandatory)")
     L = C.CDLL(path)
