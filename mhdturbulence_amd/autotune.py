@@ -14,6 +14,7 @@ import itertools
 import json
 import os
 
+from .capi import MhdError
 from .solver import Box, Config
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiling_cache.json")
@@ -40,10 +41,13 @@ def tune(nx: int, ny: int, nz: int, cfg: Config | None = None, steps: int = 4, c
     box.run(2, mode=0)
     results = []
     for sw, ly, lz in cands or candidates(nx, ny, nz):
-        box.set_tiling(sw, ly, lz)
+        try:
+            box.set_tiling(sw, ly, lz)
+        except MhdError:  # beyond the xy kernel's LDS for this strip width
+            continue
         box.profile_kernels(1, mode=0)  # first launch of this tiling outside the timing
         prof = box.profile_kernels(steps, mode=0)
-        ms = prof["fused_edge_x"] + prof["fused_xy"] + prof["fused_z_update"]
+        ms = sum(v for k, v in prof.items() if k != "step_end")
         results.append({"sw": sw, "ly": ly, "lz": lz, "ms": ms})
         if log:
             log(f"sw={sw} ly={ly} lz={lz}: {ms:.4f} ms")

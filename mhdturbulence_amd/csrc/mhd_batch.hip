@@ -1,6 +1,6 @@
 // mhd_batch.hip -- many boxes of different sizes and parameters advanced together.
 //
-// Each step issues one launch per fused kernel (edge, xy, z) plus one step-end launch for
+// Each step issues one launch per fused kernel (xy, z) plus one step-end launch for
 // all live boxes: a work table maps every block to (box, block of that box), and each box
 // keeps its own dt / c_h ring (a segmented reduction: a box's bounds never mix with another
 // box's), its own time and stop flag.  The per-step launches are captured in a graph that
@@ -51,7 +51,6 @@ struct BatchBox {
   double* U0 = nullptr;
   double* U1 = nullptr;
   double* acc = nullptr;
-  double* edge = nullptr;
   double* grid[6] = {};
   double* diag = nullptr;
   FusedArgs fa{};
@@ -73,14 +72,14 @@ Geom make_geom(int nx, int ny, int nz) {
 uint64_t up(uint64_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
 
 // byte sizes of a box's arrays, in carve order
-void box_layout(const Geom& g, uint64_t edge_elems, uint64_t* sz) {
+constexpr int NREGION = 11;
+void box_layout(const Geom& g, uint64_t* sz) {
   sz[0] = up(mhd::NPRIM * g.vstride * 8);                  // P
   sz[1] = sz[2] = sz[3] = up(mhd::NCONS * g.vstride * 8);  // U0 U1 acc
-  sz[4] = up(edge_elems * 8);                              // strip-edge fluxes
-  sz[5] = sz[6] = up((uint64_t)g.itot * 8);                // x1a x1b
-  sz[7] = sz[8] = up((uint64_t)g.jtot * 8);                // x2a x2b
-  sz[9] = sz[10] = up((uint64_t)g.ktot * 8);               // x3a x3b
-  sz[11] = up(4ull * g.nz * 8);                            // diagnostics partials
+  sz[4] = sz[5] = up((uint64_t)g.itot * 8);                // x1a x1b
+  sz[6] = sz[7] = up((uint64_t)g.jtot * 8);                // x2a x2b
+  sz[8] = sz[9] = up((uint64_t)g.ktot * 8);                // x3a x3b
+  sz[10] = up(4ull * g.nz * 8);                            // diagnostics partials
 }
 
 }  // namespace
@@ -98,9 +97,9 @@ struct mhd_batch {
   StepScalars* d_sc = nullptr;
   FusedArgs* d_args = nullptr;
   double* d_dt_last = nullptr;
-  int2* d_tab[3] = {nullptr, nullptr, nullptr};
-  int tab_cap[3] = {0, 0, 0};
-  int nblk[3] = {0, 0, 0};
+  int2* d_tab[2] = {nullptr, nullptr};  // work tables of the xy and z launches
+  int tab_cap[2] = {0, 0};
+  int nblk[2] = {0, 0};
   int* d_live = nullptr;
   int nlive = 0;
   bool dirty = true;
@@ -113,10 +112,11 @@ namespace {
 
 long long shape_key(int nx, int ny, int nz) { return ((long long)nx << 40) | ((long long)ny << 20) | nz; }
 
-FusedArgs configure(mhd_batch* b, const Geom& g) {
+FusedArgs configure(mhd_batch* b, const Geom& g, bool* ok) {
   FusedArgs a{};
   a.g = g;
-  mhd::fused_configure(a);
+  a.mixed = b->base.precision;
+  *ok = mhd::fused_configure(a);
   auto it = b->tilings.find(shape_key(g.nx, g.ny, g.nz));
   if (it != b->tilings.end()) {
     a.sw = it->second.sw;
@@ -175,17 +175,19 @@ int ensure_tab(mhd_batch* b, int q, int n) {
 
 // work tables and the captured step graph for the current set of live boxes
 int rebuild(mhd_batch* b) {
-  std::vector<int2> tab[3];
+  std::vector<int2> tab[2];
   std::vector<int> live;
+  size_t lds = 0;
   for (int s = 0; s < b->max_boxes; ++s) {
     const BatchBox& x = b->boxes[s];
     if (!x.live) continue;
     live.push_back(s);
-    const int nb[3] = {mhd::fused_blocks_edge(x.fa), mhd::fused_blocks_xy(x.fa), mhd::fused_blocks_z(x.fa)};
-    for (int q = 0; q < 3; ++q)
+    const int nb[2] = {mhd::fused_blocks_xy(x.fa), mhd::fused_blocks_z(x.fa)};
+    for (int q = 0; q < 2; ++q)
       for (int k = 0; k < nb[q]; ++k) tab[q].push_back(make_int2(s, k));
+    lds = std::max(lds, mhd::fused_xy_lds(x.fa));
   }
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < 2; ++q) {
     BCHK(ensure_tab(b, q, std::max<int>(1, (int)tab[q].size())));
     if (!tab[q].empty())
       BHIP(hipMemcpy(b->d_tab[q], tab[q].data(), tab[q].size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -196,9 +198,9 @@ int rebuild(mhd_batch* b) {
   drop_graph(b);
   if (b->nlive > 0) {
     BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    mhd::launch_batch_edge(b->d_args, b->d_tab[0], b->nblk[0], b->base.energy_flux, b->base.precision, b->stream);
-    mhd::launch_batch_xy(b->d_args, b->d_tab[1], b->nblk[1], b->base.energy_flux, b->base.precision, b->stream);
-    mhd::launch_batch_z(b->d_args, b->d_tab[2], b->nblk[2], b->base.energy_flux, b->base.precision, b->stream);
+    mhd::launch_batch_xy(b->d_args, b->d_tab[0], b->nblk[0], lds, b->base.energy_flux, b->base.precision,
+                         b->stream);
+    mhd::launch_batch_z(b->d_args, b->d_tab[1], b->nblk[1], b->base.energy_flux, b->base.precision, b->stream);
     mhd::launch_batch_step_end(b->d_args, b->d_live, b->nlive, b->d_dt_last, b->stream);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &gr);
@@ -241,7 +243,7 @@ int mhd_batch_destroy(mhd_batch* b) {
   (void)hipSetDevice(b->dev);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  for (int q = 0; q < 3; ++q)
+  for (int q = 0; q < 2; ++q)
     if (b->d_tab[q]) (void)hipFree(b->d_tab[q]);
   void* bufs[] = {b->arena, b->d_sc, b->d_args, b->d_dt_last, b->d_live};
   for (void* p : bufs)
@@ -291,9 +293,10 @@ int mhd_batch_create(const mhd_config* base, int max_boxes, uint64_t arena_bytes
 int mhd_batch_box_bytes(const mhd_batch* b, int nx, int ny, int nz, uint64_t* bytes) {
   if (!b || !bytes || nx < 4 || ny < 4 || nz < 4) return fail(MHD_ERR_ARG, "bad box size");
   const Geom g = make_geom(nx, ny, nz);
-  FusedArgs a = configure(const_cast<mhd_batch*>(b), g);
-  uint64_t sz[12];
-  box_layout(g, mhd::fused_edge_elems(a), sz);
+  bool fits = false;  // the byte count is reported either way; add_box rejects what does not fit
+  (void)configure(const_cast<mhd_batch*>(b), g, &fits);
+  uint64_t sz[NREGION];
+  box_layout(g, sz);
   uint64_t t = 0;
   for (uint64_t v : sz) t += v;
   *bytes = t;
@@ -312,8 +315,11 @@ int mhd_batch_free_bytes(const mhd_batch* b, uint64_t* largest, uint64_t* total)
 }
 
 int mhd_batch_set_tiling(mhd_batch* b, int nx, int ny, int nz, int sw, int ly, int lz) {
-  if (!b || (sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > ny || lz > nz)
-    return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= nz");
+  if (!b || (sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > ny || lz > nz || lz > 64)
+    return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= min(nz, 64)");
+  if (ly > mhd::fused_max_ly(sw, b->base.precision))
+    return fail(MHD_ERR_ARG, "tiling: ly above " + std::to_string(mhd::fused_max_ly(sw, b->base.precision)) +
+                                 " (LDS of the xy kernel) for this strip width and precision");
   b->tilings[shape_key(nx, ny, nz)] = Tiling{sw, ly, lz};
   return MHD_OK;
 }
@@ -333,18 +339,20 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
   BatchBox& x = b->boxes[slot];
   x = BatchBox{};
   x.g = make_geom(nx, ny, nz);
-  FusedArgs a = configure(b, x.g);
-  uint64_t sz[12];
-  box_layout(x.g, mhd::fused_edge_elems(a), sz);
+  bool fits = false;
+  FusedArgs a = configure(b, x.g, &fits);
+  if (!fits) return fail(MHD_ERR_ARG, "box too large: a variable column spans 4 GiB or more");
+  uint64_t sz[NREGION];
+  box_layout(x.g, sz);
   uint64_t total = 0;
   for (uint64_t v : sz) total += v;
   if (!arena_take(b, total, &x.off)) return fail(MHD_ERR_NOMEM, "arena has no free region that large");
   x.bytes = total;
   char* base = b->arena + x.off;
-  double** dst[12] = {&x.P, &x.U0, &x.U1, &x.acc, &x.edge, &x.grid[0], &x.grid[1], &x.grid[2], &x.grid[3],
-                      &x.grid[4], &x.grid[5], &x.diag};
+  double** dst[NREGION] = {&x.P,       &x.U0,      &x.U1,      &x.acc,     &x.grid[0], &x.grid[1],
+                           &x.grid[2], &x.grid[3], &x.grid[4], &x.grid[5], &x.diag};
   uint64_t o = 0;
-  for (int q = 0; q < 12; ++q) {
+  for (int q = 0; q < NREGION; ++q) {
     *dst[q] = reinterpret_cast<double*>(base + o);
     o += sz[q];
   }
@@ -368,7 +376,6 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
   a.U0 = x.U0;
   a.U1 = x.U1;
   a.acc = x.acc;
-  a.edge = x.edge;
   a.sc = sc;
   a.gam = b->base.gam;
   a.cfl = b->base.cfl;

@@ -73,9 +73,8 @@ struct mhd_ctx {
   hipGraphExec_t graph = nullptr;
   int graph_mode = -1;
   double* graph_log = nullptr;
-  // fused loop buffers: second conserved copy (U2 above), flux divergence, strip-edge fluxes
+  // fused loop buffers: second conserved copy (U2 above), flux divergence
   double* acc = nullptr;
-  double* edge = nullptr;
   mhd::FusedArgs fa{};
   bool fa_ready = false;
 };
@@ -105,17 +104,18 @@ int ensure_fused(mhd_ctx* c) {
   if (c->fa_ready) return MHD_OK;
   mhd::FusedArgs& a = c->fa;
   a.g = c->g;
-  mhd::fused_configure(a);
+  a.mixed = c->cfg.precision;
+  if (!mhd::fused_configure(a))
+    return fail(MHD_ERR_UNSUPPORTED, "fused step: a variable column of this box spans 4 GiB or more "
+                                     "(more than 2^29 cells); use the staged step (mode 1)");
   CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
   CHKRET(dalloc(c, &c->acc, mhd::NCONS * c->g.vstride));
-  CHKRET(dalloc(c, &c->edge, mhd::fused_edge_elems(a)));
   a.x1a = c->grid[0];
   a.x2a = c->grid[2];
   a.x3a = c->grid[4];
   a.U0 = c->U;
   a.U1 = c->U2;
   a.acc = c->acc;
-  a.edge = c->edge;
   a.sc = c->sc;
   a.gam = c->cfg.gam;
   a.cfl = c->cfg.cfl;
@@ -193,7 +193,7 @@ int mhd_destroy(mhd_ctx* c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->acc, c->edge, c->staging, c->diag, c->dt_log,
+  double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->acc, c->staging, c->diag, c->dt_log,
                     c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5]};
   for (double* b : bufs)
     if (b) hipFree(b);
@@ -410,8 +410,6 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
     ++q;
   };
   if (mode == 0) {
-    mhd::launch_fused_edge(c->fa, c->stream);
-    mark();
     mhd::launch_fused_xy(c->fa, c->stream);
     mark();
     mhd::launch_fused_z(c->fa, c->stream);
@@ -441,11 +439,11 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
 }
 
 static const char* kernel_names(int mode) {
-  if (mode == 0) return "fused_edge_x,fused_xy,fused_z_update,step_end";
+  if (mode == 0) return "fused_xy,fused_z_update,step_end";
   return "reduce_dt_ch,boundary,flux_x,flux_y,flux_z,update_damp,update_prim,step_end";
 }
 
-static int launches_per_step(int mode) { return mode == 0 ? 4 : 8; }
+static int launches_per_step(int mode) { return mode == 0 ? 3 : 8; }
 
 // loop state and buffers for nsteps iterations starting at *time
 static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
@@ -561,25 +559,19 @@ const char* mhd_kernel_names(int mode) { return kernel_names(mode); }
 
 int mhd_set_tiling(mhd_ctx* c, int sw, int ly, int lz) {
   if (!c) return fail(MHD_ERR_ARG, "null context");
-  if ((sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > c->g.ny || lz > c->g.nz)
-    return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= nz");
+  if ((sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > c->g.ny || lz > c->g.nz ||
+      lz > 64)
+    return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= min(nz, 64)");
+  if (ly > mhd::fused_max_ly(sw, c->cfg.precision))
+    return fail(MHD_ERR_ARG, "tiling: ly above " + std::to_string(mhd::fused_max_ly(sw, c->cfg.precision)) +
+                                 " (LDS of the xy kernel) for this strip width and precision");
   CHKRET(set_dev(c));
   CHKRET(ensure_fused(c));
-  const int ns = (c->g.nx + sw - 1) / sw;
   mhd::FusedArgs a = c->fa;
   a.sw = sw;
-  a.ns = ns;
+  a.ns = (c->g.nx + sw - 1) / sw;
   a.ly = ly;
   a.lz = lz;
-  if (mhd::fused_edge_elems(a) > mhd::fused_edge_elems(c->fa)) {
-    // the strip-edge buffer grows with the number of strips
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->bytes -= mhd::fused_edge_elems(c->fa) * sizeof(double);
-    (void)hipFree(c->edge);
-    c->edge = nullptr;
-    CHKRET(dalloc(c, &c->edge, mhd::fused_edge_elems(a)));
-    a.edge = c->edge;
-  }
   c->fa = a;
   drop_graph(c);
   return MHD_OK;

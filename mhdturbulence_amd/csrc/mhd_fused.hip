@@ -1,10 +1,10 @@
 // mhd_fused.hip -- fused step kernels for gfx950 (see mhd_fused.hpp for the data flow).
 //
 // Per loop iteration of src_cpp_kokkos_host/main.cpp:229-259 the fused path reads the
-// conserved state once in each of the xy and z kernels (plus the x-strip edges), keeps
-// primitives, slopes, interface states and fluxes in registers, and writes the flux
-// divergence once and the new state once.  Every face flux is computed exactly once
-// (the strip-edge x faces by the edge kernel), every limited slope once per cell and
+// conserved state once in each of the xy and z kernels (plus the x-strip edge cells),
+// keeps primitives, slopes, interface states and fluxes in registers, and writes the flux
+// divergence once and the new state once.  Every face flux is computed once (the x faces
+// on strip edges once per adjacent strip), every limited slope once per cell and
 // direction, as the reference's operation sequence allows: the same slope of cell f
 // serves the right state of face f-1/2 and the left state of face f+1/2
 // (KH/mhd.cpp:1134-1154).
@@ -24,12 +24,6 @@ constexpr int WAVES = 4;  // wavefronts per block; blocks carry independent wave
 #ifndef MHD_F64_MINWAVES
 #define MHD_F64_MINWAVES 1
 #endif
-// MHD_PREFETCH=1: the march loads the raw state of the row two ahead, and the z kernel
-// issues the loads of the cell it updates before that iteration's Riemann solve, so the
-// load latency overlaps arithmetic even at one wavefront per SIMD.
-#ifndef MHD_PREFETCH
-#define MHD_PREFETCH 1
-#endif
 template <typename R>
 struct MinWaves {
   static constexpr int value = sizeof(R) == 8 ? MHD_F64_MINWAVES : 2;
@@ -38,20 +32,60 @@ struct MinWaves {
 __device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigned long long)__double_as_longlong(d); }
 
+// A variable column is addressed as a wave-uniform base held in scalar registers plus a
+// 32-bit byte offset, the scalar-base form of global loads and stores, so no load or store
+// pays 64-bit vector address arithmetic.  The base goes through readfirstlane so the compiler
+// keeps it scalar instead of folding the column offset into a per-lane 64-bit address.
+// fused_configure rejects boxes with vstride * 8 >= 2^32.
+__device__ __forceinline__ unsigned boff(int c) { return (unsigned)c * 8u; }
+typedef __attribute__((address_space(1))) char gchar;  // global-memory bytes
+__device__ __forceinline__ gchar* sbase(const double* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (gchar*)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double ldc(const double* __restrict__ A, size_t vs, int m, unsigned ob) {
+  return *(const __attribute__((address_space(1))) double*)(sbase(A + (size_t)m * vs) + ob);
+}
+__device__ __forceinline__ void stc(double* __restrict__ A, size_t vs, int m, unsigned ob, double v) {
+  *(__attribute__((address_space(1))) double*)(sbase(A + (size_t)m * vs) + ob) = v;
+}
+
+// The flux divergence acc passed from the xy to the z kernel is cell-major, NCONS doubles
+// per cell at acc + c * NCONS: one 64-bit address per lane and five 16-byte accesses with
+// immediate offsets (rows of a wavefront still cover one contiguous span).
+__device__ __forceinline__ void store_acc(double* __restrict__ acc, int c, const double* v) {
+  double2* p = reinterpret_cast<double2*>(acc + (size_t)c * NCONS);
+#pragma unroll
+  for (int m = 0; m < NCONS / 2; ++m) p[m] = make_double2(v[2 * m], v[2 * m + 1]);
+}
+__device__ __forceinline__ void load_acc(const double* __restrict__ acc, int c, double* v) {
+  const double2* p = reinterpret_cast<const double2*>(acc + (size_t)c * NCONS);
+#pragma unroll
+  for (int m = 0; m < NCONS / 2; ++m) {
+    const double2 t = p[m];
+    v[2 * m] = t.x;
+    v[2 * m + 1] = t.y;
+  }
+}
+
 // conserved state of one cell -> primitive vector in precision R (UpdatePrimitvP arithmetic in FP64)
 template <typename R>
 __device__ __forceinline__ void load_prim(const double* __restrict__ U, size_t vs, int c, double gam, R* q) {
   double u[NCONS], p[NPRIM];
+  const unsigned ob = boff(c);
 #pragma unroll
-  for (int m = 0; m < NCONS; ++m) u[m] = U[m * vs + c];
+  for (int m = 0; m < NCONS; ++m) u[m] = ldc(U, vs, m, ob);
   cons_to_prim(u, gam, p);
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) q[n] = R(p[n]);
 }
 
 __device__ __forceinline__ void load_raw(const double* __restrict__ U, size_t vs, int c, double* u) {
+  const unsigned ob = boff(c);
 #pragma unroll
-  for (int m = 0; m < NCONS; ++m) u[m] = U[m * vs + c];
+  for (int m = 0; m < NCONS; ++m) u[m] = ldc(U, vs, m, ob);
 }
 
 template <typename R>
@@ -65,6 +99,40 @@ __device__ __forceinline__ void recover(const double* u, double gam, R* q) {
 template <typename R>
 __device__ __forceinline__ R lane_get(R v, int src) {
   return __shfl(v, src, 64);
+}
+
+// Primitive recovery of the row/plane loaded at the start of a march pass, placed after
+// the pass's Riemann solves and before its stores: the scheduling barrier keeps the
+// arithmetic from being hoisted next to the loads (which would wait on them early), and the
+// pinned results keep it from being sunk below the stores (a wait there drains the stores).
+template <typename R>
+__device__ __forceinline__ void recover_in_place(const double* u, double gam, R* q) {
+  __builtin_amdgcn_sched_barrier(0);
+  recover<R>(u, gam, q);
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) asm volatile("" : "+v"(q[n]));
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// lane r's value of v (r wave-uniform): a per-row constant kept one per lane.  Call it in
+// wave-uniform control flow on a value from pin_lanes, so that every lane has computed v.
+__device__ __forceinline__ double pin_lanes(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ double lane_row(double v, int r) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, r);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), r);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// a value the compiler must hold in registers as is: an LDS read passed through here stays
+// an unconditional read feeding a lane select, instead of becoming a branch per value
+template <typename T>
+__device__ __forceinline__ T in_reg(T v) {
+  asm("" : "+v"(v));
+  return v;
 }
 
 // source of ghost gg (0,1) on side (0 lo, 1 hi) of an axis with interior [lo, hi]
@@ -92,56 +160,32 @@ __device__ __forceinline__ void write_images(double* __restrict__ Uo, const Geom
       if (ghost_src(bc[2 * ax], bc[2 * ax + 1], side, gg, lo, hi, neg) != c) continue;
       int t[3] = {i, j, k};
       t[ax] = side ? hi + 1 + gg : gg;
-      const int cc = g.cell(t[2], t[1], t[0]);
+      const unsigned oc = boff(g.cell(t[2], t[1], t[0]));
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) Uo[m * g.vstride + cc] = (neg && m == 1 + ax) ? -u[m] : u[m];
+      for (int m = 0; m < NCONS; ++m) stc(Uo, g.vstride, m, oc, (neg && m == 1 + ax) ? -u[m] : u[m]);
     }
   }
 }
 
-__device__ __forceinline__ size_t edge_index(const FusedArgs& a, int m, int k, int j, int e) {
-  return (((size_t)m * a.g.nz + (k - NGH)) * a.g.ny + (j - NGH)) * (a.ns + 1) + e;
-}
-
-// ---------------------------------------------------------------- edge x faces
-// x faces at the left edge of each strip and the right edge of the last one:
-// face index f = NGH + min(e*sw, nx), e in [0, ns]; one thread per (e, j, k).
-template <int EFLUX, typename R>
-__device__ __forceinline__ void edge_body(const FusedArgs& a, long long t) {
-  const StepScalars* sc = a.sc;
-  if (sc->stopped) return;
-  const Geom& g = a.g;
-  const int iter = sc->iter;
-  const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
-  const double chg = bits2d(sc->red[iter % 3][1]);
-  const int ne = a.ns + 1;
-  if (t >= (long long)ne * g.ny * g.nz) return;
-  const int e = (int)(t % ne);
-  const long long r = t / ne;
-  const int j = NGH + (int)(r % g.ny);
-  const int k = NGH + (int)(r / g.ny);
-  const int f = NGH + min(e * a.sw, g.nx);
-  R q[4][NPRIM];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) load_prim<R>(U, g.vstride, g.cell(k, j, f - 2 + s), a.gam, q[s]);
-  R ql[NPRIM], qr[NPRIM], tmp[NPRIM];
-  cell_states<R>(q[0], q[1], q[2], ql, tmp);
-  cell_states<R>(q[1], q[2], q[3], tmp, qr);
-  double F[NCONS];
-  riemann_flux<0, EFLUX, R>(ql, qr, chg, 0.5 / chg, F);
-#pragma unroll
-  for (int m = 0; m < NCONS; ++m) a.edge[edge_index(a, m, k, j, e)] = F[m];
-}
-
-template <int EFLUX, typename R>
-__global__ __launch_bounds__(256) void k_fused_edge(FusedArgs a) {
-  edge_body<EFLUX, R>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 // ---------------------------------------------------------------- x + y divergence
-// one wavefront: x-strip s of sw cells (64/sw strips stacked in k), rows [j0, j1) of chunk yc
+// One wavefront: x-strip s of sw cells (64/sw strips stacked in k), rows [j0, j1) of
+// chunk yc.  Lanes hold one cell each; x neighbours come through cross-lane shuffles.
+// The two x faces on the strip's edges need cells of the neighbouring strips: a pre-pass
+// computes them for every row of the chunk (one face per lane) and parks the face fluxes
+// and the outside neighbour cells in this wave's LDS slots, so the row march itself only
+// loads the conserved state of the row two ahead.  A face shared by two strips is computed
+// by both with the same instructions on the same inputs, so both see the same bits.
+//
+// LDS of one wave: F[slot][NCONS] doubles, then N[slot][NPRIM] of R,
+// slot = (sub * 2 + side) * ly + r  (side 0: left edge face, 1: right edge face; r = row - j0)
+template <typename R>
+__host__ __device__ constexpr size_t xy_slot_bytes() { return NCONS * sizeof(double) + NPRIM * sizeof(R); }
+
+template <typename R>
+__host__ __device__ inline size_t xy_wave_lds(int sw, int ly) { return (size_t)(64 / sw) * 2 * ly * xy_slot_bytes<R>(); }
+
 template <int EFLUX, typename R>
-__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane) {
+__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -175,43 +219,68 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane) {
   const int src_r = sub * SW + (l < SW - 1 ? l + 1 : l);
   const int j0 = NGH + yc * a.ly;
   const int j1 = min(j0 + a.ly, NGH + g.ny);
+  const int nrows = j1 - j0;
   const double xinv = 1.0 / (a.x1a[i + 1] - a.x1a[i]);
+  // 1/dy of the chunk's rows, row j0 + r in lane r (a.ly <= 64), read back per row by index
+  const double yinv_l =
+      pin_lanes(1.0 / (a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)]));
+  double* ldsF = reinterpret_cast<double*>(lds);
+  R* ldsN = reinterpret_cast<R*>(lds + (size_t)SPW * 2 * a.ly * NCONS * sizeof(double));
+
+  // pre-pass: the strip-edge x faces of every row of the chunk, one face per lane
+  const int nf = SPW * 2 * nrows;
+  for (int f = lane; f < nf; f += 64) {
+    const int r = f % nrows, side = (f / nrows) & 1, sb = f / (2 * nrows);
+    const int kr = min(NGH + kg * SPW + sb, NGH + g.nz - 1);
+    const int fc = side ? ilast + 1 : i0;  // the face between cells fc-1 and fc
+    R q[4][NPRIM];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) load_prim<R>(U, vs, g.cell(kr, j0 + r, fc - 2 + c), a.gam, q[c]);
+    R ql[NPRIM], qr[NPRIM], unused[NPRIM];
+    cell_states<R>(q[0], q[1], q[2], ql, unused);
+    cell_states<R>(q[1], q[2], q[3], unused, qr);
+    double F[NCONS];
+    riemann_flux<0, EFLUX, R>(ql, qr, chg, hchg, F);
+    const int slot = (sb * 2 + side) * a.ly + r;
+#pragma unroll
+    for (int m = 0; m < NCONS; ++m) ldsF[slot * NCONS + m] = F[m];
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) ldsN[slot * NPRIM + n] = side ? q[2][n] : q[1][n];
+  }
+  // LDS traffic of one wavefront is ordered; the fence keeps the compiler from moving the
+  // slot reads of the march above the writes
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
   R Pm[NPRIM], P0[NPRIM], Pp[NPRIM], upprev[NPRIM];
   double Fyprev[NCONS];
   load_prim<R>(U, vs, g.cell(k, j0 - 2, i), a.gam, Pm);
   load_prim<R>(U, vs, g.cell(k, j0 - 1, i), a.gam, P0);
+  load_prim<R>(U, vs, g.cell(k, j0, i), a.gam, Pp);
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fyprev[m] = 0.0;
-#if MHD_PREFETCH
-  double un[NCONS];
-  load_raw(U, vs, g.cell(k, j0, i), un);
-#endif
 
+  // Pass j: the window holds rows j-1, j, j+1.  The raw state of row j+2 is loaded at the
+  // start and turned into primitives after both Riemann solves have covered the load
+  // latency; only computed values cross the loop edge, so no load in flight is copied.
   for (int j = j0 - 1; j <= j1; ++j) {
-#if MHD_PREFETCH
-    recover<R>(un, a.gam, Pp);
+    double un[NCONS];
     load_raw(U, vs, g.cell(k, j + 2, i), un);  // row j+2 <= ny+4 < jtot
-#else
-    load_prim<R>(U, vs, g.cell(k, j + 1, i), a.gam, Pp);
-#endif
     const int row = j - 1;  // the cell row completed in this pass
     const bool out_row = row >= j0;
     double ax[NCONS];
     if (out_row) {
-      // x neighbours of row `row` (= Pm): shuffles inside the strip, loads at its edges
-      R nb1[NPRIM], nb2[NPRIM];
-      if (is_left || is_right) load_prim<R>(U, vs, g.cell(k, row, is_left ? i - 1 : i + 1), a.gam, nb1);
-      if (is_left && is_right) load_prim<R>(U, vs, g.cell(k, row, i + 1), a.gam, nb2);
+      const int sl = (sub * 2) * a.ly + (row - j0), sr = sl + a.ly;
       R qm[NPRIM], qp[NPRIM];
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) {
         const R vm = lane_get(Pm[n], src_l);
         const R vp = lane_get(Pm[n], src_r);
-        qm[n] = is_left ? nb1[n] : vm;
-        qp[n] = is_right ? (is_left ? nb2[n] : nb1[n]) : vp;
+        const R nl = in_reg(ldsN[sl * NPRIM + n]);
+        const R nr = in_reg(ldsN[sr * NPRIM + n]);
+        qm[n] = is_left ? nl : vm;
+        qp[n] = is_right ? nr : vp;
       }
       R rt[NPRIM], lt[NPRIM];
       cell_states<R>(qm, Pm, qp, rt, lt);
@@ -220,36 +289,40 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane) {
       for (int n = 0; n < NPRIM; ++n) rtl[n] = lane_get(rt[n], src_l);
       double Fl[NCONS];
       riemann_flux<0, EFLUX, R>(rtl, lt, chg, hchg, Fl);
-      if (is_left) {
 #pragma unroll
-        for (int m = 0; m < NCONS; ++m) Fl[m] = a.edge[edge_index(a, m, k, row, s)];
-      }
-      double Fr[NCONS];
-#pragma unroll
-      for (int m = 0; m < NCONS; ++m) Fr[m] = lane_get(Fl[m], src_r);
-      if (is_right) {
-#pragma unroll
-        for (int m = 0; m < NCONS; ++m) Fr[m] = a.edge[edge_index(a, m, k, row, s + 1)];
+      for (int m = 0; m < NCONS; ++m) {
+        const double fl = in_reg(ldsF[sl * NCONS + m]);
+        Fl[m] = is_left ? fl : Fl[m];
       }
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) ax[m] = (Fr[m] - Fl[m]) * xinv;
+      for (int m = 0; m < NCONS; ++m) {
+        const double fr = lane_get(Fl[m], src_r);
+        const double fe = in_reg(ldsF[sr * NCONS + m]);
+        ax[m] = ((is_right ? fe : fr) - Fl[m]) * xinv;
+      }
     }
+    const double yinv = lane_row(yinv_l, max(row - j0, 0));
     // y states of row j and the y face j-1/2
     R up[NPRIM], dn[NPRIM];
     cell_states<R>(Pm, P0, Pp, up, dn);
     double Fy[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (j >= j0) riemann_flux<1, EFLUX, R>(upprev, dn, chg, hchg, Fy);
+    // row j+2 enters before this pass's stores are issued: a wait for a load behind a
+    // pending store would drain the store too (one counter for both on this architecture)
+    R Pn[NPRIM];
+    recover_in_place<R>(un, a.gam, Pn);
     if (out_row && active) {
-      const double yinv = 1.0 / (a.x2a[row + 1] - a.x2a[row]);
-      const int c = g.cell(k, row, i);
+      double d[NCONS];
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) a.acc[m * vs + c] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
+      for (int m = 0; m < NCONS; ++m) d[m] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
+      store_acc(a.acc, g.cell(k, row, i), d);
     }
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) {
       upprev[n] = up[n];
       Pm[n] = P0[n];
       P0[n] = Pp[n];
+      Pp[n] = Pn[n];
     }
 #pragma unroll
     for (int m = 0; m < NCONS; ++m) Fyprev[m] = Fy[m];
@@ -258,7 +331,9 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane) {
 
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
-  xy_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+  extern __shared__ __align__(16) char smem[];
+  const int wv = threadIdx.x >> 6;
+  xy_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
@@ -291,65 +366,61 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
   const int k1 = min(k0 + a.lz, NGH + g.nz);
   const double dxa = a.x1a[i + 1] - a.x1a[i];
   const double dya = a.x2a[j + 1] - a.x2a[j];
+  // dz and 1/dz of the chunk's planes, plane k0 + r in lane r (a.lz <= 64)
+  const int kl = k0 + min(lane, k1 - k0 - 1);
+  const double dz_l = pin_lanes(a.x3a[kl + 1] - a.x3a[kl]);
+  const double zinv_l = pin_lanes(1.0 / (a.x3a[kl + 1] - a.x3a[kl]));
+  // ghost images exist only for cells within two of a face: a wave whose x-strip and row
+  // are away from the x and y faces skips write_images except on the first/last two planes
+  const bool img_xy = s * 64 <= 1 || NGH + (s + 1) * 64 - 1 >= NGH + g.nx - 2 || jj <= 1 || jj >= g.ny - 2;
 
   R Pm[NPRIM], P0[NPRIM], Pp[NPRIM], upprev[NPRIM];
   double Fzprev[NCONS];
   load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, Pm);
   load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, P0);
+  load_prim<R>(U, vs, g.cell(k0, j, i), a.gam, Pp);
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fzprev[m] = 0.0;
   double dtl = __builtin_inf(), chl = 0.0;
-#if MHD_PREFETCH
-  double un[NCONS];
-  load_raw(U, vs, g.cell(k0, j, i), un);
-#endif
 
+  // Pass k: the window holds planes k-1, k, k+1 and cell k-1 is updated.  The raw state of
+  // plane k+2 and the old state and divergence of cell k-1 are loaded at the start; plane
+  // k+2 becomes primitives after the Riemann solve (only computed values cross the loop edge).
   for (int k = k0 - 1; k <= k1; ++k) {
     const int kk = k - 1;  // the cell updated in this pass
     const bool upd = kk >= k0 && active;
-#if MHD_PREFETCH
-    recover<R>(un, a.gam, Pp);
+    double un[NCONS];
     load_raw(U, vs, g.cell(k + 2, j, i), un);  // plane k+2 <= nz+4 < ktot
     double uo[NCONS], ac[NCONS];
-    if (upd) {
-      const int c = g.cell(kk, j, i);
+    const unsigned oc = boff(g.cell(kk, j, i));
+    const double dza = lane_row(dz_l, max(kk - k0, 0));
+    const double zinv = lane_row(zinv_l, max(kk - k0, 0));
+    // issued by every lane on every pass (cell kk >= k0 - 2 >= 0 is in the array; unused
+    // when not updated), so the number of loads in flight is the same on every path and a
+    // wait for an earlier load need not drain these
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) {
-        uo[m] = U[m * vs + c];
-        ac[m] = a.acc[m * vs + c];
-      }
-    }
-#else
-    load_prim<R>(U, vs, g.cell(k + 1, j, i), a.gam, Pp);
-#endif
+    for (int m = 0; m < NCONS; ++m) uo[m] = ldc(U, vs, m, oc);
+    load_acc(a.acc, g.cell(kk, j, i), ac);
     R up[NPRIM], dn[NPRIM];
     cell_states<R>(Pm, P0, Pp, up, dn);
     double Fz[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, hchg, Fz);
+    // plane k+2 enters before this pass's stores are issued (see the xy march)
+    R Pn[NPRIM];
+    recover_in_place<R>(un, a.gam, Pn);
     if (upd) {
-      const double dza = a.x3a[kk + 1] - a.x3a[kk];
-      const double zinv = 1.0 / dza;
-      const int c = g.cell(kk, j, i);
       double u[NCONS];
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) {
-#if MHD_PREFETCH
-        const double div = ac[m] + (Fz[m] - Fzprev[m]) * zinv;
-        u[m] = uo[m] - dt * div;
-#else
-        const double div = a.acc[m * vs + c] + (Fz[m] - Fzprev[m]) * zinv;
-        u[m] = U[m * vs + c] - dt * div;
-#endif
-      }
+      for (int m = 0; m < NCONS; ++m) u[m] = uo[m] - dt * (ac[m] + (Fz[m] - Fzprev[m]) * zinv);
       // DampPsi (KH/mhd.cpp:1625-1631)
       const double dhl = fmin(dxa, fmin(dya, dza));
       const double taui = a.alphabp * chg / dhl;
       u[mbps] = u[mbps] * (1.0 - dt * taui);
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) Uo[m * vs + c] = u[m];
-      write_images(Uo, g, a.bc, i, j, kk, u);
+      for (int m = 0; m < NCONS; ++m) stc(Uo, vs, m, oc, u[m]);
+      if (img_xy || kk <= NGH + 1 || kk >= NGH + g.nz - 2) write_images(Uo, g, a.bc, i, j, kk, u);
       // the next iteration's ControlTimestep / EvaluateCh bounds of this cell
       double q[NPRIM];
       cons_to_prim(u, a.gam, q);
@@ -361,6 +432,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
       upprev[n] = up[n];
       Pm[n] = P0[n];
       P0[n] = Pp[n];
+      Pp[n] = Pn[n];
     }
 #pragma unroll
     for (int m = 0; m < NCONS; ++m) Fzprev[m] = Fz[m];
@@ -371,7 +443,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
     dtl = fmin(dtl, __shfl_xor(dtl, o, 64));
     chl = fmax(chl, __shfl_xor(chl, o, 64));
   }
-  const unsigned long long anybad = __ballot(bad);
+  // every wave updates some cells, so an infinite wave minimum means no finite bound (NaN state)
+  const unsigned long long anybad = __ballot(bad) | (isinf(dtl) ? 1ull : 0ull);
   if (lane == 0) {
     unsigned long long* nx = sc->red[(iter + 1) % 3];
     if (!(chl == chl)) chl = __builtin_inf();
@@ -391,18 +464,13 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
 // block count is a whole number of blocks, so a block never straddles two boxes and the
 // box descriptor read is uniform over the block.
 template <int EFLUX, typename R>
-__global__ __launch_bounds__(256) void k_batch_edge(const FusedArgs* __restrict__ boxes, const int2* __restrict__ tab) {
-  const int2 e = tab[blockIdx.x];
-  const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  edge_body<EFLUX, R>(a, (long long)e.y * blockDim.x + threadIdx.x);
-}
-
-template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
                                                                      const int2* __restrict__ tab) {
+  extern __shared__ __align__(16) char smem[];
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  xy_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  xy_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
 }
 
 template <int EFLUX, typename R>
@@ -437,7 +505,7 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 
 template <int EFLUX, typename R>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fused_xy<EFLUX, R>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), 0, s, a);
+  hipLaunchKernelGGL((k_fused_xy<EFLUX, R>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s, a);
 }
 
 template <int EFLUX, typename R>
@@ -445,24 +513,16 @@ void launch_z_t(const FusedArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_fused_z<EFLUX, R>), dim3(fused_blocks_z(a)), dim3(64 * WAVES), 0, s, a);
 }
 
-template <int EFLUX, typename R>
-void launch_edge_t(const FusedArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fused_edge<EFLUX, R>), dim3(fused_blocks_edge(a)), dim3(256), 0, s, a);
-}
-
 struct BatchLaunch {
   const FusedArgs* boxes;
   const int2* tab;
   int nblocks;
+  size_t lds;
 };
 
 template <int EFLUX, typename R>
-void launch_bedge_t(const BatchLaunch& b, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_edge<EFLUX, R>), dim3(b.nblocks), dim3(256), 0, s, b.boxes, b.tab);
-}
-template <int EFLUX, typename R>
 void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), 0, s, b.boxes, b.tab);
+  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab);
 }
 template <int EFLUX, typename R>
 void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
@@ -481,7 +541,16 @@ void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
 
 }  // namespace
 
-void fused_configure(FusedArgs& a) {
+int fused_max_ly(int sw, int mixed) {
+  const size_t per_row = (size_t)WAVES * (mixed ? xy_wave_lds<float>(sw, 1) : xy_wave_lds<double>(sw, 1));
+  return (int)(FUSED_XY_LDS_MAX / per_row);
+}
+
+size_t fused_xy_lds(const FusedArgs& a) {
+  return (size_t)WAVES * (a.mixed ? xy_wave_lds<float>(a.sw, a.ly) : xy_wave_lds<double>(a.sw, a.ly));
+}
+
+bool fused_configure(FusedArgs& a) {
   // strip width with the fewest idle lanes (ties: the wider strip)
   int best = 64, best_lanes = 1 << 30;
   for (int sw : {64, 32, 16}) {
@@ -490,13 +559,10 @@ void fused_configure(FusedArgs& a) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
-  a.ly = std::min(32, a.g.ny);
+  a.ly = std::min({32, a.g.ny, fused_max_ly(best, a.mixed)});
   a.lz = std::min(32, a.g.nz);
+  return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
-
-size_t fused_edge_elems(const FusedArgs& a) { return (size_t)NCONS * a.g.nz * a.g.ny * (a.ns + 1); }
-
-int fused_blocks_edge(const FusedArgs& a) { return (int)cdiv((long long)(a.ns + 1) * a.g.ny * a.g.nz, 256); }
 
 int fused_blocks_xy(const FusedArgs& a) {
   const int spw = 64 / a.sw;
@@ -509,21 +575,18 @@ int fused_blocks_z(const FusedArgs& a) {
   return (int)cdiv(waves, WAVES);
 }
 
-void launch_batch_edge(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bedge_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
-}
-void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
+void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, size_t lds, int eflux, int mixed,
+                     hipStream_t s) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, lds}), s);
 }
 void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks}), s);
+  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, 0}), s);
 }
 void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s) {
   if (nlive > 0)
     hipLaunchKernelGGL(k_batch_step_end, dim3(cdiv(nlive, 256)), dim3(256), 0, s, boxes, live, nlive, dt_last);
 }
 
-void launch_fused_edge(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_edge_t, a, s); }
 void launch_fused_xy(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_xy_t, a, s); }
 void launch_fused_z(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_z_t, a, s); }
 

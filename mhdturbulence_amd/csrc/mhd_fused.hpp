@@ -1,9 +1,9 @@
 // mhd_fused.hpp -- the fused step: three kernels per loop iteration instead of eight.
 //
-//   edge : x-face fluxes on the boundaries between x-strips (one thread per face)
 //   xy   : x- and y-face fluxes and their divergence  acc = dFx/dx + dFy/dy
 //          (a wavefront holds a 64/32/16-cell x-strip in its lanes; x neighbours come
-//          through cross-lane shuffles; it marches in y keeping a 3-row register window)
+//          through cross-lane shuffles, the strip-edge faces from a per-chunk pre-pass
+//          parked in LDS; it marches in y keeping a 3-row register window)
 //   z    : z-face fluxes marching in z, then U_new = U - dt*(acc + dFz/dz), psi damping,
 //          primitive recovery of U_new, the next iteration's dt / c_h reduction, and the
 //          ghost cells of U_new (periodic / reflecting / outflow images)
@@ -24,8 +24,7 @@ struct FusedArgs {
   const double* x3a;
   double* U0;      // iteration iter reads U[iter & 1] and writes the other copy
   double* U1;
-  double* acc;     // NCONS * vstride
-  double* edge;    // NCONS * nz * ny * (ns + 1)
+  double* acc;     // NCONS * vstride, cell-major: acc[c * NCONS + m]
   StepScalars* sc;
   double gam, cfl, alphabp;
   int bc[6];
@@ -35,23 +34,28 @@ struct FusedArgs {
   int eflux, mixed;
 };
 
-// strip width, chunk lengths and buffer sizes for a geometry
-void fused_configure(FusedArgs& a);
-size_t fused_edge_elems(const FusedArgs& a);
+// strip width and chunk lengths for a geometry (a.g and a.mixed set); false if a variable
+// column of the box spans 4 GiB or more (the fused kernels address columns with 32-bit offsets)
+bool fused_configure(FusedArgs& a);
+
+// LDS of the xy kernel: per block (WAVES = 4 waves) for a box's tiling, and the longest
+// y chunk a strip width allows within FUSED_XY_LDS_MAX bytes per block
+constexpr size_t FUSED_XY_LDS_MAX = 64 * 1024;
+size_t fused_xy_lds(const FusedArgs& a);
+int fused_max_ly(int sw, int mixed);
 
 // blocks of 256 threads each kernel needs for one box
-int fused_blocks_edge(const FusedArgs& a);
 int fused_blocks_xy(const FusedArgs& a);
 int fused_blocks_z(const FusedArgs& a);
 
-void launch_fused_edge(const FusedArgs& a, hipStream_t s);
 void launch_fused_xy(const FusedArgs& a, hipStream_t s);
 void launch_fused_z(const FusedArgs& a, hipStream_t s);
 
 // Batched launches: boxes[] are device-resident descriptors, tab[block] = (box, block of
 // that box); nblocks = tab length.  All boxes of a batch share eflux and precision.
-void launch_batch_edge(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
-void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
+// lds: dynamic LDS bytes of the xy launch, the largest fused_xy_lds over its boxes
+void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, size_t lds, int eflux, int mixed,
+                     hipStream_t s);
 void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
 // time += dt etc. for each listed box (live[] holds box indices); dt_last[box] gets its dt
 void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s);

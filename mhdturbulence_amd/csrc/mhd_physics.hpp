@@ -24,11 +24,13 @@ __host__ __device__ constexpr int slot(int c, int d) { return (c - d + 3) % 3; }
 // Cartesian component sitting in rotated slot s for direction D
 __host__ __device__ constexpr int comp(int s, int d) { return (s + d) % 3; }
 
-// Division of the hot paths.  MHD_FAST_DIV=1 (default): FP64 uses the hardware reciprocal
-// refined by two Newton steps and one residual correction (error within an ulp or two of
-// the IEEE quotient, no special-case scaling); FP32 uses the hardware reciprocal.
-// Operands are finite and bounded away from zero wherever the step divides (densities,
-// wave-speed differences guarded by +-1e-20, sums of same-sign slopes).
+// Division of the hot paths.  MHD_FAST_DIV=1 (default): FP64 uses the hardware reciprocal,
+// one Newton step and one residual correction.  With r0 = (1/b)(1+e) the Newton step leaves
+// a relative error e^2 in r, and the correction q + r (a - b q) squares that again, so for
+// any reciprocal seed better than 2^-14 the quotient is within an ulp of the IEEE one (no
+// special-case scaling).  FP32 uses the hardware reciprocal.  Operands are finite and
+// bounded away from zero wherever the step divides (densities, wave-speed differences
+// guarded by +-1e-20, sums of same-sign slopes).
 #ifndef MHD_FAST_DIV
 #define MHD_FAST_DIV 1
 #endif
@@ -41,13 +43,42 @@ template <>
 __device__ __forceinline__ double dv<double>(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
   r = fma(fma(-b, r, 1.0), r, r);
-  r = fma(fma(-b, r, 1.0), r, r);
   const double q = a * r;
   return fma(fma(-b, q, a), r, q);
 }
 template <>
 __device__ __forceinline__ float dv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
+}
+#endif
+
+// Square root of the hot paths.  MHD_FAST_SQRT=1 (default): FP64 from the hardware
+// reciprocal square root y, one Goldschmidt step (g = x y, h = y/2, r = 1/2 - h g) and one
+// residual correction g + (x - g^2) h: the error of y is squared twice, leaving the result
+// within an ulp of the IEEE root.  It skips the denormal/huge-argument rescaling, which the
+// step's arguments (sound and Alfven speeds squared, densities) never need; x = 0 returns 0
+// and x < 0 returns NaN as sqrt does.  FP32 uses the hardware square root.
+#ifndef MHD_FAST_SQRT
+#define MHD_FAST_SQRT 1
+#endif
+template <typename R>
+__device__ __forceinline__ R sq(R x) {
+  return sqrt(x);
+}
+#if MHD_FAST_SQRT
+template <>
+__device__ __forceinline__ double sq<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  g = fma(fma(-g, g, x), h, g);
+  return x == 0.0 ? x : g;
+}
+template <>
+__device__ __forceinline__ float sq<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
 }
 #endif
 
@@ -107,7 +138,7 @@ __device__ __forceinline__ void prim_to_state(const R* q, RState<R>& s) {
   s.fsc = rho * q[nst] * vn;
   const R css = q[ncsp] * q[ncsp];
   const R cts = css + bsq * rhoinv;
-  s.cf = sqrt((cts + sqrt(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  s.cf = sq((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
   s.vn = vn;
   s.pt = pt;
   s.vv = vf;
@@ -205,8 +236,8 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
                   mins1 * eer;
 
   // double-star states (KH/mhd.cpp:337-377)
-  const R sqrtrol = sqrt(rolst);
-  const R sqrtror = sqrt(rorst);
+  const R sqrtrol = sq(rolst);
+  const R sqrtror = sq(rorst);
   const R abbx = fabs(bxs);
   const R signbx = copysign1(bxs);
   sign1 = copysign1(abbx - eps);
@@ -314,7 +345,7 @@ __device__ __forceinline__ void cons_to_prim(const double* u, double gam, double
   const double e = (u[meto] - ekin - emag) * rhoinv;
   q[nene] = e;
   q[npre] = e * rho * (gam - 1.0);
-  q[ncsp] = sqrt(e * gam * (gam - 1.0));
+  q[ncsp] = sq(e * gam * (gam - 1.0));
   q[nbm1] = u[mbm1];
   q[nbm2] = u[mbm2];
   q[nbm3] = u[mbm3];
@@ -325,7 +356,7 @@ __device__ __forceinline__ void cons_to_prim(const double* u, double gam, double
 // CFL time-step bound of one cell (KH/mhd.cpp:1542-1553) with the cell widths given
 __device__ __forceinline__ double cell_dt(const double* q, double dx, double dy, double dz) {
   const double ctot =
-      sqrt(q[ncsp] * q[ncsp] + dv(q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3], q[nden]));
+      sq(q[ncsp] * q[ncsp] + dv(q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3], q[nden]));
   const double d1 = dv(dx, fabs(q[nve1]) + ctot);
   const double d2 = dv(dy, fabs(q[nve2]) + ctot);
   const double d3 = dv(dz, fabs(q[nve3]) + ctot);
@@ -341,9 +372,9 @@ __device__ __forceinline__ double cell_ch(const double* q) {
   const double ca2 = q[nbm2] * q[nbm2] * rinv;
   const double ca3 = q[nbm3] * q[nbm3] * rinv;
   const double cts = css + ca1 + ca2 + ca3;
-  const double cm1 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca1)) / 2.0);
-  const double cm2 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca2)) / 2.0);
-  const double cm3 = sqrt((cts + sqrt(cts * cts - 4.0 * css * ca3)) / 2.0);
+  const double cm1 = sq((cts + sq(cts * cts - 4.0 * css * ca1)) / 2.0);
+  const double cm2 = sq((cts + sq(cts * cts - 4.0 * css * ca2)) / 2.0);
+  const double cm3 = sq((cts + sq(cts * cts - 4.0 * css * ca3)) / 2.0);
   return fmax(fabs(q[nve1]) + cm1, fmax(fabs(q[nve2]) + cm2, fabs(q[nve3]) + cm3));
 }
 
