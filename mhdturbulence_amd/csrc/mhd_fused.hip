@@ -127,6 +127,24 @@ __device__ __forceinline__ double lane_row(double v, int r) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// DPP wave shifts (the whole 64-lane wave): lane l receives v from lane l-1 (SHR) or l+1
+// (SHL); lane 0 (SHR) / lane 63 (SHL) keep `old`.
+#ifndef MHD_XY_DPP
+#define MHD_XY_DPP 1
+#endif
+constexpr int DPP_WAVE_SHR1 = 0x138, DPP_WAVE_SHL1 = 0x130;
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v, double old) {
+  const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v, float old) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
 // a value the compiler must hold in registers as is: an LDS read passed through here stays
 // an unconditional read feeding a lane select, instead of becoming a branch per value
 template <typename T>
@@ -184,7 +202,9 @@ __host__ __device__ constexpr size_t xy_slot_bytes() { return NCONS * sizeof(dou
 template <typename R>
 __host__ __device__ inline size_t xy_wave_lds(int sw, int ly) { return (size_t)(64 / sw) * 2 * ly * xy_slot_bytes<R>(); }
 
-template <int EFLUX, typename R>
+// WIDE: every strip is one full wave (sw = 64, nx a multiple of 64); x neighbours then move
+// by DPP wave shifts, and the strip-edge values from LDS enter as the shifts' end-lane values.
+template <int EFLUX, typename R, bool WIDE>
 __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
@@ -275,18 +295,26 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
       R qm[NPRIM], qp[NPRIM];
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) {
-        const R vm = lane_get(Pm[n], src_l);
-        const R vp = lane_get(Pm[n], src_r);
         const R nl = in_reg(ldsN[sl * NPRIM + n]);
         const R nr = in_reg(ldsN[sr * NPRIM + n]);
-        qm[n] = is_left ? nl : vm;
-        qp[n] = is_right ? nr : vp;
+        if constexpr (WIDE) {
+          qm[n] = dpp_shift<DPP_WAVE_SHR1>(Pm[n], nl);
+          qp[n] = dpp_shift<DPP_WAVE_SHL1>(Pm[n], nr);
+        } else {
+          const R vm = lane_get(Pm[n], src_l);
+          const R vp = lane_get(Pm[n], src_r);
+          qm[n] = is_left ? nl : vm;
+          qp[n] = is_right ? nr : vp;
+        }
       }
       R rt[NPRIM], lt[NPRIM];
       cell_states<R>(qm, Pm, qp, rt, lt);
       R rtl[NPRIM];
 #pragma unroll
-      for (int n = 0; n < NPRIM; ++n) rtl[n] = lane_get(rt[n], src_l);
+      for (int n = 0; n < NPRIM; ++n) {
+        if constexpr (WIDE) rtl[n] = dpp_shift<DPP_WAVE_SHR1>(rt[n], rt[n]);  // lane 0's face comes from LDS
+        else rtl[n] = lane_get(rt[n], src_l);
+      }
       double Fl[NCONS];
       riemann_flux<0, EFLUX, R>(rtl, lt, chg, hchg, Fl);
 #pragma unroll
@@ -296,9 +324,13 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
       }
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) {
-        const double fr = lane_get(Fl[m], src_r);
         const double fe = in_reg(ldsF[sr * NCONS + m]);
-        ax[m] = ((is_right ? fe : fr) - Fl[m]) * xinv;
+        if constexpr (WIDE) {
+          ax[m] = (dpp_shift<DPP_WAVE_SHL1>(Fl[m], fe) - Fl[m]) * xinv;
+        } else {
+          const double fr = lane_get(Fl[m], src_r);
+          ax[m] = ((is_right ? fe : fr) - Fl[m]) * xinv;
+        }
       }
     }
     const double yinv = lane_row(yinv_l, max(row - j0, 0));
@@ -329,17 +361,21 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   }
 }
 
-template <int EFLUX, typename R>
+// strips fill whole waves (the DPP neighbour path applies)
+__host__ __device__ inline bool xy_wide(const FusedArgs& a) { return MHD_XY_DPP && a.sw == 64 && a.g.nx % 64 == 0; }
+
+template <int EFLUX, typename R, bool WIDE>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
   extern __shared__ __align__(16) char smem[];
   const int wv = threadIdx.x >> 6;
-  xy_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
+  xy_body<EFLUX, R, WIDE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
+constexpr int Z_RING = 4 * NPRIM * 64;  // LDS ring of one wave, in values of R
 // one wavefront: 64 x-cells at row j, planes [k0, k1) of chunk zc
 template <int EFLUX, typename R>
-__device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
+__device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring) {
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -374,20 +410,40 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
   // are away from the x and y faces skips write_images except on the first/last two planes
   const bool img_xy = s * 64 <= 1 || NGH + (s + 1) * 64 - 1 >= NGH + g.nx - 2 || jj <= 1 || jj >= g.ny - 2;
 
-  R Pm[NPRIM], P0[NPRIM], Pp[NPRIM], upprev[NPRIM];
-  double Fzprev[NCONS];
-  load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, Pm);
-  load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, P0);
-  load_prim<R>(U, vs, g.cell(k0, j, i), a.gam, Pp);
+  // The window planes and the previous pass's upper interface state live in this wave's LDS
+  // ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1 in
+  // rotation (sm, s0, sp) and slot 3 the upper state of plane k-1.  A plane is written once
+  // and read once per pass instead of being copied through registers, which leaves the
+  // registers to the Riemann solve.
+  auto rdr = [&](int slot, R* q) {
 #pragma unroll
-  for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
+    for (int n = 0; n < NPRIM; ++n) q[n] = ring[(slot * NPRIM + n) * 64 + lane];
+  };
+  auto wr = [&](int slot, const R* q) {
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) ring[(slot * NPRIM + n) * 64 + lane] = q[n];
+  };
+  {
+    R q[NPRIM];
+    load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, q);
+    wr(0, q);
+    load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, q);
+    wr(1, q);
+    load_prim<R>(U, vs, g.cell(k0, j, i), a.gam, q);
+    wr(2, q);
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) q[n] = R(0);
+    wr(3, q);
+  }
+  int sm = 0, s0 = 1, sp = 2;
+  double Fzprev[NCONS];
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fzprev[m] = 0.0;
   double dtl = __builtin_inf(), chl = 0.0;
 
-  // Pass k: the window holds planes k-1, k, k+1 and cell k-1 is updated.  The raw state of
-  // plane k+2 and the old state and divergence of cell k-1 are loaded at the start; plane
-  // k+2 becomes primitives after the Riemann solve (only computed values cross the loop edge).
+  // Pass k: cell k-1 is updated.  The raw state of plane k+2 and the old state and
+  // divergence of cell k-1 are loaded at the start; plane k+2 becomes primitives after the
+  // Riemann solve (only computed values cross the loop edge).
   for (int k = k0 - 1; k <= k1; ++k) {
     const int kk = k - 1;  // the cell updated in this pass
     const bool upd = kk >= k0 && active;
@@ -404,12 +460,22 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
     for (int m = 0; m < NCONS; ++m) uo[m] = ldc(U, vs, m, oc);
     load_acc(a.acc, g.cell(kk, j, i), ac);
     R up[NPRIM], dn[NPRIM];
-    cell_states<R>(Pm, P0, Pp, up, dn);
+    {
+      R Pm[NPRIM], P0[NPRIM], Pp[NPRIM];
+      rdr(sm, Pm);
+      rdr(s0, P0);
+      rdr(sp, Pp);
+      cell_states<R>(Pm, P0, Pp, up, dn);
+    }
+    R upprev[NPRIM];
+    rdr(3, upprev);
+    wr(3, up);
     double Fz[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, hchg, Fz);
     // plane k+2 enters before this pass's stores are issued (see the xy march)
     R Pn[NPRIM];
     recover_in_place<R>(un, a.gam, Pn);
+    wr(sm, Pn);
     if (upd) {
       double u[NCONS];
 #pragma unroll
@@ -427,13 +493,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
       dtl = fmin(dtl, cell_dt(q, dxa, dya, dza));
       chl = fmax(chl, cell_ch(q));
     }
-#pragma unroll
-    for (int n = 0; n < NPRIM; ++n) {
-      upprev[n] = up[n];
-      Pm[n] = P0[n];
-      P0[n] = Pp[n];
-      Pp[n] = Pn[n];
-    }
+    const int t0 = sm;
+    sm = s0;
+    s0 = sp;
+    sp = t0;
 #pragma unroll
     for (int m = 0; m < NCONS; ++m) Fzprev[m] = Fz[m];
   }
@@ -456,7 +519,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane) {
 
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
-  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+  __shared__ R zring[WAVES * Z_RING];
+  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING);
 }
 
 // ---------------------------------------------------------------- batched launches
@@ -470,7 +534,9 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
   const int wv = threadIdx.x >> 6;
-  xy_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
+  char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
+  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds);
+  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds);
 }
 
 template <int EFLUX, typename R>
@@ -478,7 +544,8 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
                                                                     const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63);
+  __shared__ R zring[WAVES * Z_RING];
+  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING);
 }
 
 // step end of every live box (k_step_end with the fused ring offset), one thread per box
@@ -505,7 +572,11 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 
 template <int EFLUX, typename R>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fused_xy<EFLUX, R>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s, a);
+  if (xy_wide(a))
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s, a);
+  else
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, false>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s,
+                       a);
 }
 
 template <int EFLUX, typename R>
