@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # summary and commands in profiles/fused_fp64_256cube_pmc.md.  Keyed by
 # (n, mode, precision, kernel); a workload without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 995389.90625 * 1024 * 2.0 + 1310720.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1320251.2 * 1024 * 2.0 + 1310721.0 * 1024 * 1.0,
 }
 
 
@@ -71,40 +71,80 @@ def cpu_baseline(n: int, steps: int) -> dict:
             "sample": f"{steps} steps of the same {n}^3 box (oracle C++/OpenMP restatement), {wall:.1f} s"}
 
 
-def serving_bench(count: int, seed: int = 20261203) -> dict:
-    """Continuous-batching server (FP32 reconstruction + HLLD, up to 48 live boxes) against the
-    plain FP64 solver serving the same stream one request at a time.
+SHEAR_BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
+                  energy_flux=1, grid_shift=0)
 
-    Requests are shear-layer boxes with n log-uniform in [32, 96] (rounded to 8) and end
-    times uniform in [0.002, 0.012] (a few tens of steps each at CFL factor 0.05; the
+
+def serving_bench(count: int, seed: int = 20261203) -> dict:
+    """Continuous-batching server against the plain FP64 solver serving the same stream one
+    request at a time.
+
+    Requests are shear-layer boxes with n log-uniform in [32, 256] (rounded to 8) and end
+    times uniform in [0.002, 0.012] (tens to a hundred steps each at CFL factor 0.05; the
     physical t_end of [1, 6] would need 1e4-1e5 forward-Euler steps per request).  The
-    one-at-a-time capacity is measured first with every request queued at t = 0; the stream
-    is then replayed with Poisson arrivals at 1x and 4x that capacity."""
+    one-at-a-time FP64 capacity is measured first with every request queued at t = 0; the
+    stream is then replayed with Poisson arrivals at 0.25, 0.5, 1 and 2 times that capacity
+    through the batched server in FP64 and in mixed precision (FP32 reconstruction + HLLD),
+    up to 48 live boxes, and through the one-at-a-time server."""
     from mhdturbulence_amd import Config
     from mhdturbulence_amd.serve import Server, poisson_stream
-    base = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
-                energy_flux=1, grid_shift=0)
-    kw = dict(n_range=(32, 96), t_range=(0.002, 0.012), seed=seed)
-    out = {"requests": count, "n_range": [32, 96], "t_end_range": [0.002, 0.012]}
+    kw = dict(n_range=(32, 256), t_range=(0.002, 0.012), seed=seed)
+    out = {"requests": count, "n_range": [32, 256], "t_end_range": [0.002, 0.012]}
     cap_stream = poisson_stream(rate=0.0, count=count, **kw)
-    single = Server(Config(**base, precision=0), max_boxes=1, arena_bytes=4 << 30, chunk_steps=10)
+    single = Server(Config(**SHEAR_BASE, precision=0), max_boxes=1, arena_bytes=8 << 30, chunk_steps=10)
     cap = single.serve(cap_stream)
     capacity = count / cap["wall_s"]
     out["fp64_one_at_a_time_capacity"] = {"requests_per_s": capacity, "cell_updates_per_s": cap["cell_updates_per_s"]}
-    for mult in (1.0, 4.0):
+    keys = ("cell_updates_per_s", "p50_latency_s", "p99_latency_s", "wall_s")
+    for mult in (0.25, 0.5, 1.0, 2.0):
         stream = poisson_stream(rate=mult * capacity, count=count, **kw)
-        one = single.serve(stream)
-        batched = Server(Config(**base, precision=1), max_boxes=48, arena_bytes=32 << 30, chunk_steps=10)
-        bat = batched.serve(stream)
-        batched.close()
-        out[f"rate_{mult:g}x"] = {
-            "arrival_rate_per_s": mult * capacity,
-            "batched_mixed": {k: bat[k] for k in ("cell_updates_per_s", "p50_latency_s", "p99_latency_s", "wall_s")},
-            "fp64_one_at_a_time": {k: one[k] for k in ("cell_updates_per_s", "p50_latency_s", "p99_latency_s",
-                                                      "wall_s")},
-            "throughput_ratio": bat["cell_updates_per_s"] / one["cell_updates_per_s"],
-        }
+        row = {"arrival_rate_per_s": mult * capacity, "fp64_one_at_a_time": {k: v for k, v in single.serve(stream).items()
+                                                                           if k in keys}}
+        for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
+            srv = Server(Config(**SHEAR_BASE, precision=prec), max_boxes=48, arena_bytes=48 << 30, chunk_steps=10)
+            r = srv.serve(stream)
+            srv.close()
+            row[name] = {k: r[k] for k in keys}
+        row["p99_ratio_batched_fp64"] = row["fp64_one_at_a_time"]["p99_latency_s"] / row["batched_fp64"]["p99_latency_s"]
+        out[f"rate_{mult:g}x"] = row
     single.close()
+    return out
+
+
+def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203) -> dict:
+    """Batched ensemble: `count` boxes, box b drawn by ensemble_box(b) (each axis from
+    {32, 48, 64, 96, 128}, a, Bx, amplitude, p from splitmix64(seed + b)), advanced together
+    through the batched launches in FP64 and in mixed precision, against the same boxes
+    advanced one at a time by the single-box FP64 solver.  Rates are interior cells x steps
+    over the wall time of the timed steps (boxes built and warmed up beforehand)."""
+    from mhdturbulence_amd import Batch, Box, Config
+    from mhdturbulence_amd.serve import ENSEMBLE_SIZES, ensemble_box
+    boxes = [ensemble_box(b, seed) for b in range(count)]
+    cells = sum(sh[0] * sh[1] * sh[2] for sh, _ in boxes)
+    out = {"boxes": count, "axis_sizes": list(ENSEMBLE_SIZES), "cells": cells, "steps": steps}
+    for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
+        bt = Batch(Config(**SHEAR_BASE, precision=prec), max_boxes=count, arena_bytes=48 << 30)
+        for sh, ic in boxes:
+            bt.add_box(*sh, ic=ic)
+        bt.step(2)
+        t0 = time.perf_counter()
+        gpu_ms = bt.step(steps)
+        wall = time.perf_counter() - t0
+        out[name] = {"cell_updates_per_s": cells * steps / wall, "gpu_ms_per_step": gpu_ms / steps}
+        bt.close()
+    wall = 0.0
+    for sh, ic in boxes:
+        box = Box(Config(nx=sh[0], ny=sh[1], nz=sh[2], **SHEAR_BASE))
+        box.generate_grid()
+        box.generate_problem_shear(**ic)
+        box.run(2)
+        t0 = time.perf_counter()
+        box.run(steps)
+        wall += time.perf_counter() - t0
+        box.close()
+    out["fp64_one_at_a_time"] = {"cell_updates_per_s": cells * steps / wall}
+    out["batched_fp64_speedup"] = out["batched_fp64"]["cell_updates_per_s"] / out["fp64_one_at_a_time"][
+        "cell_updates_per_s"]
     return out
 
 
@@ -120,7 +160,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-requests", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
@@ -176,6 +216,8 @@ def main():
     }
     del box
     if not args.no_serve:
+        log("[bench] batched ensemble: 96 boxes, batched FP64 / mixed vs one at a time")
+        out["ensemble"] = ensemble_bench()
         log(f"[bench] serving: {args.serve_requests} requests, batched vs one at a time")
         out["serving"] = serving_bench(args.serve_requests)
     if not args.no_cpu:

@@ -631,7 +631,11 @@ bool fused_configure(FusedArgs& a) {
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
   a.ly = std::min({32, a.g.ny, fused_max_ly(best, a.mixed)});
-  a.lz = std::min(32, a.g.nz);
+  // the longest z chunk (fewest re-read halo planes) that still gives the z kernel at least
+  // two wavefronts per SIMD of the chip (256 CUs x 4 SIMDs)
+  const long long cols = (long long)((a.g.nx + 63) / 64) * a.g.ny;
+  a.lz = std::min(64, a.g.nz);
+  while (a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
 

@@ -69,6 +69,24 @@ def request_params(rid: int, seed: int = 20261203) -> dict:
                 seed=(seed + rid) & 0xFFFFFFFFFFFFFFFF)
 
 
+ENSEMBLE_SIZES = (32, 48, 64, 96, 128)
+
+
+def ensemble_box(b: int, seed: int = 20261203) -> tuple[tuple[int, int, int], dict]:
+    """Box b of the batched ensemble: (nx, ny, nz) each drawn from ENSEMBLE_SIZES and the
+    shear-layer parameters (a, Bx, amplitude, p) from the splitmix64 sequence started at
+    seed + b: a in [0.02,0.1], Bx in [0,0.2], amplitude in [1e-3,1e-2], p in [0.5,5]"""
+    s = seed + b
+    u = []
+    for _ in range(7):
+        s = splitmix64(s)
+        u.append((s >> 11) * 2.0 ** -53)
+    shape = tuple(ENSEMBLE_SIZES[min(int(x * len(ENSEMBLE_SIZES)), len(ENSEMBLE_SIZES) - 1)] for x in u[:3])
+    ic = dict(a=0.02 + 0.08 * u[3], bx0=0.2 * u[4], amp=1e-3 + 9e-3 * u[5], p0=0.5 + 4.5 * u[6],
+              seed=(seed + b) & 0xFFFFFFFFFFFFFFFF)
+    return shape, ic
+
+
 def poisson_stream(rate: float, count: int, seed: int = 20261203, n_range=(32, 256), t_range=(1.0, 6.0),
                    multiple: int = 8) -> list[Request]:
     """Poisson arrivals at `rate` per second; n log-uniform in n_range rounded to a multiple
