@@ -153,7 +153,8 @@ int mhd_batch_destroy(mhd_batch* batch);
 /* arena bytes one box of this size takes (the admission test) */
 int mhd_batch_box_bytes(const mhd_batch* batch, int nx, int ny, int nz, uint64_t* bytes);
 int mhd_batch_free_bytes(const mhd_batch* batch, uint64_t* largest_free, uint64_t* total_free);
-/* tile shape for boxes of size nx*ny*nz added later (sw: 16/32/64 x-strip, ly/lz: march chunks) */
+/* tile shape for boxes of size nx*ny*nz added later (sw: 16/32/64 x-strip, ly/lz: march
+ * chunks; limits as mhd_set_tiling) */
 int mhd_batch_set_tiling(mhd_batch* batch, int nx, int ny, int nz, int sw, int ly, int lz);
 /* grid (GenerateGrid) and, if ic != NULL, the shear-layer state on the device; tmax <= 0: no stop */
 int mhd_batch_add_box(mhd_batch* batch, int nx, int ny, int nz, const mhd_ic_params* ic, double tmax, int* box_id);
@@ -170,7 +171,9 @@ int mhd_batch_poll(mhd_batch* batch, int* ids, int cap, int* n);
  * P_host (may be NULL) the primitive array in the reference layout */
 int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_host);
 
-/* tile shape of a single box's fused kernels (sw: 16/32/64, 1 <= ly <= ny, 1 <= lz <= nz) */
+/* tile shape of a single box's fused kernels: sw 16/32/64 (x-strip width), 1 <= ly <= ny
+ * (y chunk; also at most what the xy kernel's 64 KiB of LDS per block allows: FP64 46, 23,
+ * 11 rows and mixed 64, 32, 16 rows for sw 64, 32, 16), 1 <= lz <= min(nz, 64) (z chunk) */
 int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
 int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
 
