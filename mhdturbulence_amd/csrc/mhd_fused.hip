@@ -373,9 +373,14 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
 constexpr int Z_RING = 4 * NPRIM * 64;  // LDS ring of one wave, in values of R
+// FP64 kernels also keep the raw conserved state of the last three planes loaded: the old
+// state of the cell a pass updates was loaded three passes earlier as the plane ahead, so it
+// is read back from LDS instead of from memory (the FP32 kernels run two blocks per CU and
+// have no LDS left for it)
+constexpr int Z_RAW = 3 * NCONS * 64;
 // one wavefront: 64 x-cells at row j, planes [k0, k1) of chunk zc
 template <int EFLUX, typename R>
-__device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring) {
+__device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -414,7 +419,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   // ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1 in
   // rotation (sm, s0, sp) and slot 3 the upper state of plane k-1.  A plane is written once
   // and read once per pass instead of being copied through registers, which leaves the
-  // registers to the Riemann solve.
+  // registers to the Riemann solve.  raw[((p % 3) * NCONS + m) * 64 + lane] (FP64 only) holds
+  // the conserved state of plane p for the last three planes loaded.
   auto rdr = [&](int slot, R* q) {
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) q[n] = ring[(slot * NPRIM + n) * 64 + lane];
@@ -429,7 +435,15 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     wr(0, q);
     load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, q);
     wr(1, q);
-    load_prim<R>(U, vs, g.cell(k0, j, i), a.gam, q);
+    {
+      double u0[NCONS];
+      load_raw(U, vs, g.cell(k0, j, i), u0);
+      if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int m = 0; m < NCONS; ++m) raw[((k0 % 3) * NCONS + m) * 64 + lane] = u0[m];
+      }
+      recover<R>(u0, a.gam, q);
+    }
     wr(2, q);
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) q[n] = R(0);
@@ -456,8 +470,14 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     // issued by every lane on every pass (cell kk >= k0 - 2 >= 0 is in the array; unused
     // when not updated), so the number of loads in flight is the same on every path and a
     // wait for an earlier load need not drain these
+    const int rs = (k + 2) % 3;  // raw slot of plane k-1 (read now) and of plane k+2 (written below)
+    if constexpr (sizeof(R) == 8) {
 #pragma unroll
-    for (int m = 0; m < NCONS; ++m) uo[m] = ldc(U, vs, m, oc);
+      for (int m = 0; m < NCONS; ++m) uo[m] = raw[(rs * NCONS + m) * 64 + lane];
+    } else {
+#pragma unroll
+      for (int m = 0; m < NCONS; ++m) uo[m] = ldc(U, vs, m, oc);
+    }
     load_acc(a.acc, g.cell(kk, j, i), ac);
     R up[NPRIM], dn[NPRIM];
     {
@@ -476,6 +496,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     R Pn[NPRIM];
     recover_in_place<R>(un, a.gam, Pn);
     wr(sm, Pn);
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+      for (int m = 0; m < NCONS; ++m) raw[(rs * NCONS + m) * 64 + lane] = un[m];
+    }
     if (upd) {
       double u[NCONS];
 #pragma unroll
@@ -520,7 +544,9 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
   __shared__ R zring[WAVES * Z_RING];
-  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING);
+  __shared__ double zraw[sizeof(R) == 8 ? WAVES * Z_RAW : 1];
+  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING,
+                   zraw + (sizeof(R) == 8 ? (threadIdx.x >> 6) * Z_RAW : 0));
 }
 
 // ---------------------------------------------------------------- batched launches
@@ -545,7 +571,9 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
   __shared__ R zring[WAVES * Z_RING];
-  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING);
+  __shared__ double zraw[sizeof(R) == 8 ? WAVES * Z_RAW : 1];
+  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING,
+                   zraw + (sizeof(R) == 8 ? (threadIdx.x >> 6) * Z_RAW : 0));
 }
 
 // step end of every live box (k_step_end with the fused ring offset), one thread per box

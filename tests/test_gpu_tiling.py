@@ -1,0 +1,91 @@
+"""Tilings of the fused kernels change only how the work is cut (strip width, y and z
+chunks, DPP or shuffle neighbours, pre-pass edge faces): every tiling must give the same
+state as the oracle, the tilings must agree with each other to round-off, and tilings the
+kernels cannot hold are refused.  Also a short run of the mixed-precision accuracy tool."""
+import sys
+
+import numpy as np
+import pytest
+
+import parity
+from mhdturbulence_amd import Box, Config
+from mhdturbulence_amd.capi import MhdError
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, energy_flux=1,
+            grid_shift=0)
+IC = dict(a=0.05, bx0=0.15, amp=0.02, p0=1.3, seed=77)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def _run(shape, tiling, steps, precision=0):
+    b = Box(Config(nx=shape[0], ny=shape[1], nz=shape[2], precision=precision, **BASE))
+    b.generate_grid()
+    b.generate_problem_shear(**IC)
+    if tiling:
+        b.set_tiling(*tiling)
+    t, dts, _ = b.run(steps)
+    P = b.interior(b.download_prim())
+    got = b.get_tiling()
+    b.close()
+    return P, dts, got
+
+
+@pytest.mark.parametrize("shape", [(64, 40, 24), (80, 36, 20)])
+def test_tilings_agree_with_oracle_and_each_other(orc, shape):
+    steps = 12
+    o = orc.Oracle(nx=shape[0], ny=shape[1], nz=shape[2], **BASE)
+    o.generate_grid()
+    o.generate_problem_shear(**IC)
+    _, odts = o.run(steps)
+    ref = o.interior(o.P)
+    # (64,40,24) takes the full-wave DPP path at sw = 64; 80 cells make ragged strips at every width
+    nx, ny, nz = shape
+    tilings = [None, (64, 8, 8), (64, min(ny, 46), nz), (32, 16, 5), (16, 7, min(nz, 24)), (16, 11, 1)]
+    results = []
+    for tl in tilings:
+        say("shape", shape, "tiling", tl)
+        P, dts, got = _run(shape, tl, steps)
+        if tl:
+            assert got == tl
+        err = parity.rel_l1(P, ref).max()
+        assert err < 1e-12, (tl, err)
+        np.testing.assert_allclose(dts, odts, rtol=1e-12)
+        results.append(P)
+    for P in results[1:]:
+        assert parity.rel_l1(P, results[0]).max() < 1e-13
+
+
+def test_tiling_limits_refused():
+    b = Box(Config(nx=64, ny=64, nz=80, **BASE))
+    with pytest.raises(MhdError):
+        b.set_tiling(48, 8, 8)        # strip width not 16/32/64
+    with pytest.raises(MhdError):
+        b.set_tiling(16, 40, 8)       # y chunk beyond the xy kernel's LDS at sw = 16 in FP64
+    with pytest.raises(MhdError):
+        b.set_tiling(64, 8, 65)       # z chunk above 64 planes
+    with pytest.raises(MhdError):
+        b.set_tiling(64, 65, 8)       # y chunk above ny
+    b.set_tiling(64, 46, 64)          # the largest y chunk FP64 allows at sw = 64
+    assert b.get_tiling() == (64, 46, 64)
+    b.close()
+
+
+def test_accuracy_tool_small():
+    from mhdturbulence_amd import accuracy
+    f = accuracy.field_errors(32, 6, 0.1)
+    assert f["fp64"]["steps"] == f["mixed"]["steps"] == 6
+    assert 0.0 < f["rel_l1_rho"] < 1e-5 and 0.0 < f["rel_l1_p"] < 1e-5
+    assert abs(f["fp64"]["time"] - f["mixed"]["time"]) < 1e-6 * f["fp64"]["time"]
+    g = accuracy.growth(24, 0.2, 0.02, 0.1, fit=(0.05, 0.2))
+    assert np.isfinite(g["fp64"]["growth_rate"]) and g["growth_rate_rel_diff"] < 1e-2
