@@ -13,7 +13,7 @@ roofline: algorithmic bytes of a cell update (read + write the 10 FP64 conserved
 variables = 160 B) per launch of the dominant kernel / that kernel's average launch
 duration from HIP events on the library's stream, against 8 TB/s.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
-host's cores on 2 steps of the same 256^3 box.
+host's cores on 16 steps (about 10 s) of the same 256^3 box.
 """
 from __future__ import annotations
 
@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # summary and commands in profiles/fused_fp64_256cube_pmc.md.  Keyed by
 # (n, mode, precision, kernel); a workload without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1320251.2 * 1024 * 2.0 + 1310721.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1324557.6 * 1024 * 2.0 + 1310721.0 * 1024 * 1.0,
 }
 
 
@@ -157,7 +157,7 @@ def main():
     ap.add_argument("--mode", type=int, default=0, help="0 fused step kernels, 1 staged kernels")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 reconstruction + HLLD")
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
