@@ -658,7 +658,12 @@ bool fused_configure(FusedArgs& a) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
+  // y chunk: 32 rows (fewer re-read halo rows and pre-passes) unless the box is too small
+  // to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs), then down
+  // to 8 rows
+  const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
   a.ly = std::min({32, a.g.ny, fused_max_ly(best, a.mixed)});
+  while (a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly /= 2;
   // the longest z chunk (fewest re-read halo planes) that still gives the z kernel at least
   // two wavefronts per SIMD of the chip (256 CUs x 4 SIMDs)
   const long long cols = (long long)((a.g.nx + 63) / 64) * a.g.ny;
