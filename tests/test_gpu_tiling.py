@@ -89,3 +89,14 @@ def test_accuracy_tool_small():
     assert abs(f["fp64"]["time"] - f["mixed"]["time"]) < 1e-6 * f["fp64"]["time"]
     g = accuracy.growth(24, 0.2, 0.02, 0.1, fit=(0.05, 0.2))
     assert np.isfinite(g["fp64"]["growth_rate"]) and g["growth_rate_rel_diff"] < 1e-2
+
+
+def test_tprof_tool_small(tmp_path):
+    from mhdturbulence_amd import tprof
+    r = tprof.run(24, 0.3, 0.05, str(tmp_path), fit=(0.0, 0.3))
+    assert r["status"] == 0 and r["time_reached"] > 0.3 and r["samples"] >= 6
+    lines = (tmp_path / "t-prof.csv").read_text().splitlines()
+    assert lines[0].startswith("# Gamma=") and len(lines) == 2 + r["samples"]
+    row = [float(x) for x in lines[2].split()]
+    assert row[0] == 0.0 and row[3] == pytest.approx(1.2e-3) and 0.0 < row[1] and 0.0 < row[2] < 1e-2
+    assert np.isfinite(r["growth_rate"])
