@@ -373,10 +373,9 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
 constexpr int Z_RING = 4 * NPRIM * 64;  // LDS ring of one wave, in values of R
-// FP64 kernels also keep the raw conserved state of the last three planes loaded: the old
+// The z kernels also keep the raw conserved state of the last three planes loaded: the old
 // state of the cell a pass updates was loaded three passes earlier as the plane ahead, so it
-// is read back from LDS instead of from memory (the FP32 kernels run two blocks per CU and
-// have no LDS left for it)
+// is read back from LDS instead of from memory.
 constexpr int Z_RAW = 3 * NCONS * 64;
 // one wavefront: 64 x-cells at row j, planes [k0, k1) of chunk zc
 template <int EFLUX, typename R>
@@ -415,12 +414,15 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   // are away from the x and y faces skips write_images except on the first/last two planes
   const bool img_xy = s * 64 <= 1 || NGH + (s + 1) * 64 - 1 >= NGH + g.nx - 2 || jj <= 1 || jj >= g.ny - 2;
 
-  // The window planes and the previous pass's upper interface state live in this wave's LDS
-  // ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1 in
-  // rotation (sm, s0, sp) and slot 3 the upper state of plane k-1.  A plane is written once
-  // and read once per pass instead of being copied through registers, which leaves the
-  // registers to the Riemann solve.  raw[((p % 3) * NCONS + m) * 64 + lane] (FP64 only) holds
-  // the conserved state of plane p for the last three planes loaded.
+  // FP64: the window planes and the previous pass's upper interface state live in this
+  // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1
+  // in rotation (sm, s0, sp) and slot 3 the upper state of plane k-1.  A plane is written
+  // once and read once per pass instead of being copied through registers, which leaves the
+  // registers to the Riemann solve.  FP32 (two wavefronts per SIMD, half the LDS each) keeps
+  // them in registers (wm, w0, wp, wu).  In both, raw[((p % 3) * NCONS + m) * 64 + lane]
+  // holds the conserved state of plane p for the last three planes loaded.
+  constexpr bool PRING = sizeof(R) == 8;
+  R wm[NPRIM], w0[NPRIM], wp[NPRIM], wu[NPRIM];
   auto rdr = [&](int slot, R* q) {
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) q[n] = ring[(slot * NPRIM + n) * 64 + lane];
@@ -429,25 +431,22 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) ring[(slot * NPRIM + n) * 64 + lane] = q[n];
   };
+  load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, wm);
+  load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, w0);
   {
-    R q[NPRIM];
-    load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, q);
-    wr(0, q);
-    load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, q);
-    wr(1, q);
-    {
-      double u0[NCONS];
-      load_raw(U, vs, g.cell(k0, j, i), u0);
-      if constexpr (sizeof(R) == 8) {
+    double u0[NCONS];
+    load_raw(U, vs, g.cell(k0, j, i), u0);
 #pragma unroll
-        for (int m = 0; m < NCONS; ++m) raw[((k0 % 3) * NCONS + m) * 64 + lane] = u0[m];
-      }
-      recover<R>(u0, a.gam, q);
-    }
-    wr(2, q);
+    for (int m = 0; m < NCONS; ++m) raw[((k0 % 3) * NCONS + m) * 64 + lane] = u0[m];
+    recover<R>(u0, a.gam, wp);
+  }
 #pragma unroll
-    for (int n = 0; n < NPRIM; ++n) q[n] = R(0);
-    wr(3, q);
+  for (int n = 0; n < NPRIM; ++n) wu[n] = R(0);
+  if constexpr (PRING) {
+    wr(0, wm);
+    wr(1, w0);
+    wr(2, wp);
+    wr(3, wu);
   }
   int sm = 0, s0 = 1, sp = 2;
   double Fzprev[NCONS];
@@ -467,39 +466,47 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     const unsigned oc = boff(g.cell(kk, j, i));
     const double dza = lane_row(dz_l, max(kk - k0, 0));
     const double zinv = lane_row(zinv_l, max(kk - k0, 0));
+    const int rs = (k + 2) % 3;  // raw slot of plane k-1 (read now) and of plane k+2 (written below)
+#pragma unroll
+    for (int m = 0; m < NCONS; ++m) uo[m] = raw[(rs * NCONS + m) * 64 + lane];
     // issued by every lane on every pass (cell kk >= k0 - 2 >= 0 is in the array; unused
     // when not updated), so the number of loads in flight is the same on every path and a
     // wait for an earlier load need not drain these
-    const int rs = (k + 2) % 3;  // raw slot of plane k-1 (read now) and of plane k+2 (written below)
-    if constexpr (sizeof(R) == 8) {
-#pragma unroll
-      for (int m = 0; m < NCONS; ++m) uo[m] = raw[(rs * NCONS + m) * 64 + lane];
-    } else {
-#pragma unroll
-      for (int m = 0; m < NCONS; ++m) uo[m] = ldc(U, vs, m, oc);
-    }
     load_acc(a.acc, g.cell(kk, j, i), ac);
-    R up[NPRIM], dn[NPRIM];
-    {
+    R up[NPRIM], dn[NPRIM], upprev[NPRIM];
+    if constexpr (PRING) {
       R Pm[NPRIM], P0[NPRIM], Pp[NPRIM];
       rdr(sm, Pm);
       rdr(s0, P0);
       rdr(sp, Pp);
       cell_states<R>(Pm, P0, Pp, up, dn);
+      rdr(3, upprev);
+      wr(3, up);
+    } else {
+      cell_states<R>(wm, w0, wp, up, dn);
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) {
+        upprev[n] = wu[n];
+        wu[n] = up[n];
+      }
     }
-    R upprev[NPRIM];
-    rdr(3, upprev);
-    wr(3, up);
     double Fz[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, hchg, Fz);
     // plane k+2 enters before this pass's stores are issued (see the xy march)
     R Pn[NPRIM];
     recover_in_place<R>(un, a.gam, Pn);
-    wr(sm, Pn);
-    if constexpr (sizeof(R) == 8) {
+    if constexpr (PRING) {
+      wr(sm, Pn);
+    } else {
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) raw[(rs * NCONS + m) * 64 + lane] = un[m];
+      for (int n = 0; n < NPRIM; ++n) {
+        wm[n] = w0[n];
+        w0[n] = wp[n];
+        wp[n] = Pn[n];
+      }
     }
+#pragma unroll
+    for (int m = 0; m < NCONS; ++m) raw[(rs * NCONS + m) * 64 + lane] = un[m];
     if (upd) {
       double u[NCONS];
 #pragma unroll
@@ -543,10 +550,12 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
-  __shared__ R zring[WAVES * Z_RING];
-  __shared__ double zraw[sizeof(R) == 8 ? WAVES * Z_RAW : 1];
-  z_body<EFLUX, R>(a, blockIdx.x * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING,
-                   zraw + (sizeof(R) == 8 ? (threadIdx.x >> 6) * Z_RAW : 0));
+  // LDS: FP64 window ring (96 KiB) + raw ring (60 KiB) per block; FP32 raw ring only
+  __shared__ R zring[sizeof(R) == 8 ? WAVES * Z_RING : 1];
+  __shared__ double zraw[WAVES * Z_RAW];
+  const int wv = threadIdx.x >> 6;
+  z_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * Z_RING : 0),
+                   zraw + wv * Z_RAW);
 }
 
 // ---------------------------------------------------------------- batched launches
@@ -570,10 +579,11 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
                                                                     const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  __shared__ R zring[WAVES * Z_RING];
-  __shared__ double zraw[sizeof(R) == 8 ? WAVES * Z_RAW : 1];
-  z_body<EFLUX, R>(a, e.y * WAVES + (threadIdx.x >> 6), threadIdx.x & 63, zring + (threadIdx.x >> 6) * Z_RING,
-                   zraw + (sizeof(R) == 8 ? (threadIdx.x >> 6) * Z_RAW : 0));
+  __shared__ R zring[sizeof(R) == 8 ? WAVES * Z_RING : 1];
+  __shared__ double zraw[WAVES * Z_RAW];
+  const int wv = threadIdx.x >> 6;
+  z_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * Z_RING : 0),
+                   zraw + wv * Z_RAW);
 }
 
 // step end of every live box (k_step_end with the fused ring offset), one thread per box
