@@ -116,7 +116,7 @@ FusedArgs configure(mhd_batch* b, const Geom& g, bool* ok) {
   FusedArgs a{};
   a.g = g;
   a.mixed = b->base.precision;
-  *ok = mhd::fused_configure(a);
+  *ok = mhd::fused_configure(a, false);
   auto it = b->tilings.find(shape_key(g.nx, g.ny, g.nz));
   if (it != b->tilings.end()) {
     a.sw = it->second.sw;

@@ -135,7 +135,7 @@ int set_dev(mhd_ctx* c) {
 }
 
 void drop_graph(mhd_ctx* c) {
-  if (c->graph) hipGraphExecDestroy(c->graph);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   c->graph = nullptr;
   c->graph_mode = -1;
 }
@@ -190,17 +190,18 @@ const char* mhd_last_error(void) { return g_last_error.c_str(); }
 
 int mhd_destroy(mhd_ctx* c) {
   if (!c) return MHD_OK;
-  hipSetDevice(c->dev);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  // teardown: errors are not actionable here, every release is attempted
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->acc, c->staging, c->diag, c->dt_log,
                     c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5]};
   for (double* b : bufs)
-    if (b) hipFree(b);
-  if (c->sc) hipFree(c->sc);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (b) (void)hipFree(b);
+  if (c->sc) (void)hipFree(c->sc);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MHD_OK;
 }

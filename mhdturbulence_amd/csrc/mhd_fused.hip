@@ -659,7 +659,7 @@ size_t fused_xy_lds(const FusedArgs& a) {
   return (size_t)WAVES * (a.mixed ? xy_wave_lds<float>(a.sw, a.ly) : xy_wave_lds<double>(a.sw, a.ly));
 }
 
-bool fused_configure(FusedArgs& a) {
+bool fused_configure(FusedArgs& a, bool alone) {
   // strip width with the fewest idle lanes (ties: the wider strip)
   int best = 64, best_lanes = 1 << 30;
   for (int sw : {64, 32, 16}) {
@@ -668,17 +668,17 @@ bool fused_configure(FusedArgs& a) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
-  // y chunk: 32 rows (fewer re-read halo rows and pre-passes) unless the box is too small
-  // to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs), then down
-  // to 8 rows
+  // y chunk: 32 rows (fewer re-read halo rows and pre-passes); a box run alone that is too
+  // small to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs) goes
+  // down to 8 rows (in a batch the other boxes fill the chip)
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
   a.ly = std::min({32, a.g.ny, fused_max_ly(best, a.mixed)});
-  while (a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly /= 2;
-  // the longest z chunk (fewest re-read halo planes) that still gives the z kernel at least
-  // two wavefronts per SIMD of the chip (256 CUs x 4 SIMDs)
+  while (alone && a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly /= 2;
+  // z chunk: the longest (fewest re-read halo planes, up to 64) that, for a box run alone,
+  // still gives the z kernel two wavefronts per SIMD of the chip; 32 planes in a batch
   const long long cols = (long long)((a.g.nx + 63) / 64) * a.g.ny;
-  a.lz = std::min(64, a.g.nz);
-  while (a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
+  a.lz = std::min(alone ? 64 : 32, a.g.nz);
+  while (alone && a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
 

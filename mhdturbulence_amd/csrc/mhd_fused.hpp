@@ -34,9 +34,10 @@ struct FusedArgs {
   int eflux, mixed;
 };
 
-// strip width and chunk lengths for a geometry (a.g and a.mixed set); false if a variable
-// column of the box spans 4 GiB or more (the fused kernels address columns with 32-bit offsets)
-bool fused_configure(FusedArgs& a);
+// strip width and chunk lengths for a geometry (a.g and a.mixed set), for a box run alone or
+// in a batch; false if a variable column of the box spans 4 GiB or more (the fused kernels
+// address columns with 32-bit offsets)
+bool fused_configure(FusedArgs& a, bool alone = true);
 
 // LDS of the xy kernel: per block (WAVES = 4 waves) for a box's tiling, and the longest
 // y chunk a strip width allows within FUSED_XY_LDS_MAX bytes per block

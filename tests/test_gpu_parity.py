@@ -162,6 +162,19 @@ def test_other_boundaries(orc, bc):
         assert np.all(errP <= TOL_L1), (mode, errP)
 
 
+@pytest.mark.parametrize("shape", [(5, 4, 6), (17, 9, 7), (65, 6, 5), (33, 70, 4)])
+def test_small_and_ragged_boxes(orc, shape):
+    """one-cell and partial strips, fewer rows / planes than a chunk, several x-strips of
+    different widths per wavefront, a y chunk split into several pre-pass batches"""
+    say("small/ragged", shape)
+    for mode in (0, 1):
+        errP, errU, _, _ = _run_parity(orc, shape, 15, mode, problem=dict(bx0=0.12, a=0.08, amp=0.03),
+                                       energy_flux=1, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5),
+                                       xmax=(0.5, 0.5, 0.5), grid_shift=0)
+        assert np.all(errP <= TOL_L1), (mode, errP)
+        assert np.all(errU <= TOL_L1), (mode, errU)
+
+
 def test_tmax_stop_matches_reference_loop(orc):
     o, b = pair(orc, n=(16, 16, 16))
     dt0 = o.control_timestep()
@@ -208,3 +221,18 @@ def test_kernel_profile_runs(orc):
     for mode in (1, 0):
         prof = b.profile_kernels(3, mode=mode)
         assert len(prof) >= 2 and all(v >= 0 for v in prof.values())
+
+
+def test_kh64_tprof_long_run_matches_oracle_fixture(tmp_path):
+    """12,700 steps of the reference KH problem at 64^3 (t = 0 to 4.5): every t-prof row of
+    the fused FP64 path equals the oracle's (tests/golden/kh64_tprof.csv, written by
+    tests/golden/make_kh64_tprof.py with the same sampling) to round-off."""
+    import os
+    from mhdturbulence_amd import tprof
+    gold = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "kh64_tprof.csv"), comments="#")
+    r = tprof.run(64, 4.5, 0.05, str(tmp_path))
+    got = np.loadtxt(tmp_path / "t-prof.csv", comments="#")
+    assert r["steps"] == int(gold[-1, 0]) and got.shape[0] == gold.shape[0]
+    np.testing.assert_allclose(got[:, 0], gold[:, 1], rtol=1e-12, atol=1e-15)  # sample times
+    np.testing.assert_allclose(got[:, 1], gold[:, 2], rtol=1e-10)  # mix
+    np.testing.assert_allclose(got[:, 2], gold[:, 3], rtol=1e-10)  # sqrt(<vy^2>)
