@@ -133,8 +133,9 @@ int mhd_last_run_timing(mhd_ctx* ctx, double* kernel_ms, int* launches);
 
 /* Per-kernel timing (the phase timers of src_cpp_kokkos_device/main.cpp:293-373, as HIP
  * events): runs nsteps loop iterations of the given mode launch by launch, with an event
- * between launches, and writes the average ms of each launch position of an iteration
- * into avg_ms (capacity cap); the state advances by nsteps steps. */
+ * after every launch (recorded up front; the host waits once, at the end), and writes the
+ * average ms of each launch position of an iteration into avg_ms (capacity cap); the state
+ * advances by nsteps steps. */
 int mhd_profile_kernels(mhd_ctx* ctx, int nsteps, int mode, double* avg_ms, int cap, int* nkernels);
 /* comma-separated names of the launch positions of one iteration of the given mode */
 const char* mhd_kernel_names(int mode);

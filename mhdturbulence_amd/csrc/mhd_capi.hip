@@ -533,22 +533,25 @@ int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int ca
   if (cap < nk) return fail(MHD_ERR_ARG, "avg_ms too small");
   double t = 0.0;
   CHKRET(prepare_run(c, nsteps, 0.0, mode, t));
-  std::vector<hipEvent_t> ev(nk + 1, nullptr);
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  std::vector<double> acc(nk, 0.0);
+  // one event before the first launch and one after every launch of every step, all
+  // recorded up front: the host waits once, at the end, so the steps run back to back
+  std::vector<hipEvent_t> ev((size_t)nsteps * nk + 1, nullptr);
   int rc = MHD_OK;
-  for (int s = 0; s < nsteps && rc == MHD_OK; ++s) {
-    if (hipEventRecord(ev[0], c->stream) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventRecord"); break; }
-    rc = issue_step(c, mode, ev.data() + 1);
-    if (rc != MHD_OK) break;
-    if (hipEventSynchronize(ev[nk]) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventSynchronize"); break; }
-    for (int q = 0; q < nk; ++q) {
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
-      acc[q] += ms;
-    }
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventCreate"); break; }
+  if (rc == MHD_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventRecord");
+  for (int s = 0; s < nsteps && rc == MHD_OK; ++s) rc = issue_step(c, mode, ev.data() + 1 + (size_t)s * nk);
+  if (rc == MHD_OK && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventSynchronize");
+  std::vector<double> acc(nk, 0.0);
+  if (rc == MHD_OK)
+    for (int s = 0; s < nsteps; ++s)
+      for (int q = 0; q < nk; ++q) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ev[(size_t)s * nk + q], ev[(size_t)s * nk + q + 1]);
+        acc[q] += ms;
+      }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (rc != MHD_OK) return rc;
   CHKRET(finish_run(c, mode, nsteps, &t, nullptr, nullptr));
   for (int q = 0; q < nk; ++q) avg_ms[q] = acc[q] / nsteps;
