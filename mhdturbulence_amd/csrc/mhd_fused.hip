@@ -19,8 +19,9 @@ namespace {
 
 constexpr int WAVES = 4;  // wavefronts per block; blocks carry independent waves
 
-// FP64 march kernels need more than 256 VGPRs; one wave per SIMD lets them keep their
-// window in AGPRs instead of scratch.  The FP32 variants fit two waves per SIMD.
+// FP64 march kernels need more than 256 registers per lane; at one wave per SIMD what does
+// not fit the 256 architectural VGPRs goes to AGPRs instead of scratch.  The FP32 variants
+// fit two waves per SIMD.
 #ifndef MHD_F64_MINWAVES
 #define MHD_F64_MINWAVES 1
 #endif
