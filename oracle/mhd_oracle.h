@@ -8,6 +8,13 @@
  * Host array layout is the reference FieldArray layout: element (n,k,j,i) of an
  * array with nv variables lives at ((k*jtot + j)*nv + n)*itot + i
  * (Kokkos::View<double****,LayoutLeft>(i,n,j,k), src_cpp_kokkos_host/mhd.hpp:164-175).
+ *
+ * Pinning: tests/test_oracle.py checks every function against closed forms, invariants,
+ * independent textbook solvers and the reference's own output files.  Parity unpinned:
+ *  - energy_flux = 0 with B != 0: the (E+pt)vn - |B|^2 Bn energy flux is restated from
+ *    src_cpp_kokkos_host/mhd.cpp:1049-1050; the tests pin only its exact difference from the
+ *    physical (v.B) Bn form, since no reference run with B != 0 could be made;
+ *  - max|divB| in orc_diagnostics has no counterpart in the reference.
  */
 #ifndef MHD_ORACLE_H_
 #define MHD_ORACLE_H_
