@@ -1,14 +1,15 @@
 """Build the HIP library (libmhd_amd.so) and the C++ drop-in host library in-tree.
 
     python -m mhdturbulence_amd.build          # compile for gfx950
-    python -m mhdturbulence_amd.build -D MHD_PREFETCH=0 --out /tmp/lib_nopf.so   # a variant;
-        # load it with MHD_AMD_LIB=/tmp/lib_nopf.so
+    python -m mhdturbulence_amd.build -D MHD_XY_DPP=0 --out /tmp/lib_nodpp.so   # a variant;
+        # load it with MHD_AMD_LIB=/tmp/lib_nodpp.so
 
 Each .hip source is compiled to an object with hipcc for gfx950, then linked into
 mhdturbulence_amd/libmhd_amd.so.  The drop-in host layer (mhdturbulence_amd/host/)
 is compiled with g++ against include/mhd_amd.h into libmhd_kh.so plus the example
 driver mhd_kh_main, so that an unmodified src_cpp_kokkos_host-style step loop links
-against the reference's function names.
+against the reference's function names.  csrc/mhd_calib.hip becomes the executable
+mhd_calib (HBM stream rates and FP64 issue rate of the GPU it runs on).
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmhd_amd.so")
 KH_LIB = os.path.join(PKG, "libmhd_kh.so")
 KH_MAIN = os.path.join(PKG, "mhd_kh_main")
+CALIB = os.path.join(PKG, "mhd_calib")
 
 ARCH = os.environ.get("MHD_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -92,9 +94,20 @@ def build_host(verbose: bool = True) -> str:
     return KH_LIB
 
 
+def build_calib(verbose: bool = True) -> str:
+    """The calibration executable (stream bandwidth, FP64 issue rate)."""
+    tmp = CALIB + ".tmp"
+    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp, os.path.join(CSRC, "mhd_calib.hip")])
+    os.replace(tmp, CALIB)
+    if verbose:
+        print(f"built {CALIB}")
+    return CALIB
+
+
 def build(verbose: bool = True) -> None:
     build_hip(verbose)
     build_host(verbose)
+    build_calib(verbose)
 
 
 def main(argv=None):
@@ -102,7 +115,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="build libmhd_amd.so (and the drop-in host layer) for gfx950")
     ap.add_argument("-q", "--quiet", action="store_true")
     ap.add_argument("-D", "--define", action="append", default=[],
-                    help="compile switch NAME=VALUE for a build variant (MHD_FAST_DIV, MHD_PREFETCH, MHD_F64_MINWAVES)")
+                    help="compile switch NAME=VALUE for a build variant (MHD_FAST_DIV, MHD_FAST_SQRT, MHD_XY_DPP, "
+                         "MHD_F64_MINWAVES)")
     ap.add_argument("--out", default=LIB, help="library path; a variant elsewhere skips the host layer")
     a = ap.parse_args(argv)
     if a.define or a.out != LIB:

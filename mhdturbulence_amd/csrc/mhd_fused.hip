@@ -186,6 +186,12 @@ __device__ __forceinline__ void write_images(double* __restrict__ Uo, const Geom
   }
 }
 
+// LDS window ring of one FP64 march wave (xy and z), in values of R: 4 slots of NPRIM x 64
+constexpr int WIN_RING = 4 * NPRIM * 64;
+#ifndef MHD_XY_RING
+#define MHD_XY_RING 1
+#endif
+
 // ---------------------------------------------------------------- x + y divergence
 // One wavefront: x-strip s of sw cells (64/sw strips stacked in k), rows [j0, j1) of
 // chunk yc.  Lanes hold one cell each; x neighbours come through cross-lane shuffles.
@@ -205,8 +211,14 @@ __host__ __device__ inline size_t xy_wave_lds(int sw, int ly) { return (size_t)(
 
 // WIDE: every strip is one full wave (sw = 64, nx a multiple of 64); x neighbours then move
 // by DPP wave shifts, and the strip-edge values from LDS enter as the shifts' end-lane values.
+//
+// FP64 (MHD_XY_RING=1): the window rows and the previous pass's upper y state live in this
+// wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold rows j-1, j, j+1 in
+// rotation and slot 3 the upper state of row j-1.  A row is written once per pass and read
+// where it is used instead of being carried (and rotated) in registers, which leaves the
+// registers to the Riemann solves.  FP32 keeps the window in registers.
 template <int EFLUX, typename R, bool WIDE>
-__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds) {
+__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds, R* ring) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -272,6 +284,15 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   // slot reads of the march above the writes
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
+  constexpr bool PRING = MHD_XY_RING && sizeof(R) == 8;
+  auto rdr = [&](int slot, R* q) {
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) q[n] = ring[(slot * NPRIM + n) * 64 + lane];
+  };
+  auto wr = [&](int slot, const R* q) {
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) ring[(slot * NPRIM + n) * 64 + lane] = q[n];
+  };
   R Pm[NPRIM], P0[NPRIM], Pp[NPRIM], upprev[NPRIM];
   double Fyprev[NCONS];
   load_prim<R>(U, vs, g.cell(k, j0 - 2, i), a.gam, Pm);
@@ -281,6 +302,13 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fyprev[m] = 0.0;
+  if constexpr (PRING) {
+    wr(0, Pm);
+    wr(1, P0);
+    wr(2, Pp);
+    wr(3, upprev);
+  }
+  int sm = 0, s0 = 1, sp = 2;
 
   // Pass j: the window holds rows j-1, j, j+1.  The raw state of row j+2 is loaded at the
   // start and turned into primitives after both Riemann solves have covered the load
@@ -293,6 +321,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
     double ax[NCONS];
     if (out_row) {
       const int sl = (sub * 2) * a.ly + (row - j0), sr = sl + a.ly;
+      if constexpr (PRING) rdr(sm, Pm);
       R qm[NPRIM], qp[NPRIM];
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) {
@@ -337,25 +366,42 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
     const double yinv = lane_row(yinv_l, max(row - j0, 0));
     // y states of row j and the y face j-1/2
     R up[NPRIM], dn[NPRIM];
-    cell_states<R>(Pm, P0, Pp, up, dn);
+    if constexpr (PRING) {
+      rdr(sm, Pm);
+      rdr(s0, P0);
+      rdr(sp, Pp);
+      cell_states<R>(Pm, P0, Pp, up, dn);
+      rdr(3, upprev);
+      wr(3, up);
+    } else {
+      cell_states<R>(Pm, P0, Pp, up, dn);
+    }
     double Fy[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (j >= j0) riemann_flux<1, EFLUX, R>(upprev, dn, chg, hchg, Fy);
     // row j+2 enters before this pass's stores are issued: a wait for a load behind a
     // pending store would drain the store too (one counter for both on this architecture)
     R Pn[NPRIM];
     recover_in_place<R>(un, a.gam, Pn);
+    if constexpr (PRING) wr(sm, Pn);
     if (out_row && active) {
       double d[NCONS];
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) d[m] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
       store_acc(a.acc, g.cell(k, row, i), d);
     }
+    if constexpr (PRING) {
+      const int t0 = sm;
+      sm = s0;
+      s0 = sp;
+      sp = t0;
+    } else {
 #pragma unroll
-    for (int n = 0; n < NPRIM; ++n) {
-      upprev[n] = up[n];
-      Pm[n] = P0[n];
-      P0[n] = Pp[n];
-      Pp[n] = Pn[n];
+      for (int n = 0; n < NPRIM; ++n) {
+        upprev[n] = up[n];
+        Pm[n] = P0[n];
+        P0[n] = Pp[n];
+        Pp[n] = Pn[n];
+      }
     }
 #pragma unroll
     for (int m = 0; m < NCONS; ++m) Fyprev[m] = Fy[m];
@@ -365,20 +411,28 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
 // strips fill whole waves (the DPP neighbour path applies)
 __host__ __device__ inline bool xy_wide(const FusedArgs& a) { return MHD_XY_DPP && a.sw == 64 && a.g.nx % 64 == 0; }
 
+// static LDS of the xy kernels: the FP64 window rings (96 KiB per block); the pre-pass
+// slots are dynamic (at most FUSED_XY_LDS_MAX)
+template <typename R>
+__host__ __device__ constexpr int xy_ring_len() { return (MHD_XY_RING && sizeof(R) == 8) ? WAVES * WIN_RING : 1; }
+
 template <int EFLUX, typename R, bool WIDE>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
   extern __shared__ __align__(16) char smem[];
+  __shared__ R xring[xy_ring_len<R>()];
   const int wv = threadIdx.x >> 6;
-  xy_body<EFLUX, R, WIDE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly));
+  xy_body<EFLUX, R, WIDE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly),
+                          xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0));
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
-constexpr int Z_RING = 4 * NPRIM * 64;  // LDS ring of one wave, in values of R
 // The z kernels also keep the raw conserved state of the last three planes loaded: the old
 // state of the cell a pass updates was loaded three passes earlier as the plane ahead, so it
 // is read back from LDS instead of from memory.
 constexpr int Z_RAW = 3 * NCONS * 64;
-// one wavefront: 64 x-cells at row j, planes [k0, k1) of chunk zc
+// one wavefront: an x-strip of sw cells (the xy kernel's strip width) on 64/sw consecutive
+// rows, one column per lane, planes [k0, k1) of chunk zc.  The columns are independent, so
+// narrow boxes stack rows in a wave instead of leaving lanes idle.
 template <int EFLUX, typename R>
 __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
   StepScalars* sc = a.sc;
@@ -392,17 +446,21 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   const double dt = a.cfl * bits2d(rd[0]);
   const double chg = bits2d(rd[1]);
   const double hchg = 0.5 / chg;
-  const int nxs = (g.nx + 63) / 64;
+  const int SW = a.sw;
+  const int SPW = 64 / SW;
+  const int sub = lane / SW, l = lane - sub * SW;
+  const int nrg = (g.ny + SPW - 1) / SPW;  // row groups
   const int nzc = (g.nz + a.lz - 1) / a.lz;
-  const int s = w % nxs;
-  const int t = w / nxs;
-  const int jj = t % g.ny;
-  const int zc = t / g.ny;
+  const int s = w % a.ns;
+  const int t = w / a.ns;
+  const int jg = t % nrg;
+  const int zc = t / nrg;
   if (zc >= nzc) return;  // uniform over the wavefront
-  const int iraw = NGH + s * 64 + lane;
-  const bool active = iraw < NGH + g.nx;
-  const int i = active ? iraw : NGH + g.nx - 1;
-  const int j = NGH + jj;
+  const int iraw = NGH + s * SW + l;
+  const int jraw = NGH + jg * SPW + sub;
+  const bool active = iraw < NGH + g.nx && jraw < NGH + g.ny;
+  const int i = min(iraw, NGH + g.nx - 1);
+  const int j = min(jraw, NGH + g.ny - 1);
   const int k0 = NGH + zc * a.lz;
   const int k1 = min(k0 + a.lz, NGH + g.nz);
   const double dxa = a.x1a[i + 1] - a.x1a[i];
@@ -411,9 +469,9 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   const int kl = k0 + min(lane, k1 - k0 - 1);
   const double dz_l = pin_lanes(a.x3a[kl + 1] - a.x3a[kl]);
   const double zinv_l = pin_lanes(1.0 / (a.x3a[kl + 1] - a.x3a[kl]));
-  // ghost images exist only for cells within two of a face: a wave whose x-strip and row
+  // ghost images exist only for cells within two of a face: a wave whose x-strip and rows
   // are away from the x and y faces skips write_images except on the first/last two planes
-  const bool img_xy = s * 64 <= 1 || NGH + (s + 1) * 64 - 1 >= NGH + g.nx - 2 || jj <= 1 || jj >= g.ny - 2;
+  const bool img_xy = s * SW <= 1 || (s + 1) * SW - 1 >= g.nx - 2 || jg * SPW <= 1 || jg * SPW + SPW - 1 >= g.ny - 2;
 
   // FP64: the window planes and the previous pass's upper interface state live in this
   // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1
@@ -552,10 +610,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
   // LDS: FP64 window ring (96 KiB) + raw ring (60 KiB) per block; FP32 raw ring only
-  __shared__ R zring[sizeof(R) == 8 ? WAVES * Z_RING : 1];
+  __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * Z_RING : 0),
+  z_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                    zraw + wv * Z_RAW);
 }
 
@@ -567,12 +625,14 @@ template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
                                                                      const int2* __restrict__ tab) {
   extern __shared__ __align__(16) char smem[];
+  __shared__ R xring[xy_ring_len<R>()];
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
   const int wv = threadIdx.x >> 6;
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
-  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds);
-  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds);
+  R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
+  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring);
+  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring);
 }
 
 template <int EFLUX, typename R>
@@ -580,10 +640,10 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
                                                                     const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  __shared__ R zring[sizeof(R) == 8 ? WAVES * Z_RING : 1];
+  __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * Z_RING : 0),
+  z_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                    zraw + wv * Z_RAW);
 }
 
@@ -677,7 +737,7 @@ bool fused_configure(FusedArgs& a, bool alone) {
   while (alone && a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly /= 2;
   // z chunk: the longest (fewest re-read halo planes, up to 64) that, for a box run alone,
   // still gives the z kernel two wavefronts per SIMD of the chip; 32 planes in a batch
-  const long long cols = (long long)((a.g.nx + 63) / 64) * a.g.ny;
+  const long long cols = (long long)a.ns * ((a.g.ny + 64 / best - 1) / (64 / best));
   a.lz = std::min(alone ? 64 : 32, a.g.nz);
   while (alone && a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
@@ -690,7 +750,8 @@ int fused_blocks_xy(const FusedArgs& a) {
 }
 
 int fused_blocks_z(const FusedArgs& a) {
-  const long long waves = (long long)((a.g.nx + 63) / 64) * a.g.ny * ((a.g.nz + a.lz - 1) / a.lz);
+  const int spw = 64 / a.sw;
+  const long long waves = (long long)a.ns * ((a.g.ny + spw - 1) / spw) * ((a.g.nz + a.lz - 1) / a.lz);
   return (int)cdiv(waves, WAVES);
 }
 

@@ -65,29 +65,48 @@ template <typename R>
 __device__ __forceinline__ R sq(R x) {
   return sqrt(x);
 }
+// sq_pos: arguments known to be positive (star-region densities), no x = 0 case
+template <typename R>
+__device__ __forceinline__ R sq_pos(R x) {
+  return sqrt(x);
+}
 #if MHD_FAST_SQRT
-template <>
-__device__ __forceinline__ double sq<double>(double x) {
+__device__ __forceinline__ double sq_rsq(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = 0.5 * y;
   const double r = fma(-h, g, 0.5);
   g = fma(g, r, g);
   h = fma(h, r, h);
-  g = fma(fma(-g, g, x), h, g);
+  return fma(fma(-g, g, x), h, g);
+}
+template <>
+__device__ __forceinline__ double sq<double>(double x) {
+  const double g = sq_rsq(x);
   return x == 0.0 ? x : g;
+}
+template <>
+__device__ __forceinline__ double sq_pos<double>(double x) {
+  return sq_rsq(x);
 }
 template <>
 __device__ __forceinline__ float sq<float>(float x) {
   return __builtin_amdgcn_sqrtf(x);
 }
+template <>
+__device__ __forceinline__ float sq_pos<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
 #endif
 
-// van Leer harmonic limiter, KH/mhd.cpp:89-95
+// Half the van Leer harmonic limiter of KH/mhd.cpp:89-95: vL(dp, dm) / 2 = dp dm / (dp + dm)
+// where dp dm > 0, else 0.  Scaling by 2 is exact in binary floating point (away from the
+// subnormal range) and commutes with every rounding of the quotient, so q0 +- h equals the
+// reference's q0 +- 0.5 * (2 dp dm / (dp + dm)) bit for bit, without the two scalings.
 template <typename R>
-__device__ __forceinline__ R van_leer(R dp, R dm) {
+__device__ __forceinline__ R half_van_leer(R dp, R dm) {
   const R prod = dp * dm;
-  const R v = dv(R(2) * dp * dm, dp + dm);
-  return prod > R(0) ? v : R(0);
+  const R h = dv(prod, dp + dm);
+  return prod > R(0) ? h : R(0);
 }
 
 // One interface state in the rotated frame: the quantities HLLD reads from the
@@ -236,8 +255,8 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
                   mins1 * eer;
 
   // double-star states (KH/mhd.cpp:337-377)
-  const R sqrtrol = sq(rolst);
-  const R sqrtror = sq(rorst);
+  const R sqrtrol = sq_pos(rolst);
+  const R sqrtror = sq_pos(rorst);
   const R abbx = fabs(bxs);
   const R signbx = copysign1(bxs);
   sign1 = copysign1(abbx - eps);
@@ -326,9 +345,9 @@ template <typename R>
 __device__ __forceinline__ void cell_states(const R* qm, const R* q0, const R* qp, R* to_right, R* to_left) {
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) {
-    const R dsv = van_leer(qp[n] - q0[n], q0[n] - qm[n]);
-    to_right[n] = q0[n] + R(0.5) * dsv;
-    to_left[n] = q0[n] - R(0.5) * dsv;
+    const R h = half_van_leer(qp[n] - q0[n], q0[n] - qm[n]);
+    to_right[n] = q0[n] + h;
+    to_left[n] = q0[n] - h;
   }
 }
 
