@@ -173,8 +173,9 @@ int mhd_batch_poll(mhd_batch* batch, int* ids, int cap, int* n);
 int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_host);
 
 /* tile shape of a single box's fused kernels: sw 16/32/64 (x-strip width), 1 <= ly <= ny
- * (y chunk; also at most what the xy kernel's 64 KiB of LDS per block allows: FP64 46, 23,
- * 11 rows and mixed 64, 32, 16 rows for sw 64, 32, 16), 1 <= lz <= min(nz, 64) (z chunk) */
+ * (y chunk; also at most what the xy kernel's LDS for strip-edge faces allows per block:
+ * FP64 46, 23, 11 rows (160 KiB less its window rings) and mixed 64, 32, 16 rows (64 KiB)
+ * for sw 64, 32, 16), 1 <= lz <= min(nz, 64) (z chunk) */
 int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
 int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
 

@@ -191,6 +191,12 @@ constexpr int WIN_RING = 4 * NPRIM * 64;
 #ifndef MHD_XY_RING
 #define MHD_XY_RING 1
 #endif
+#ifndef MHD_XY_COOP
+#define MHD_XY_COOP 1
+#endif
+// block-shared strip-edge exchange of the xy kernels, per wave: edge cells of 4 rows (2 sides
+// each) and the right interface state of the last cell, in values of R; one face flux
+constexpr int XY_EDGE_R = 9 * NPRIM;
 
 // ---------------------------------------------------------------- x + y divergence
 // One wavefront: x-strip s of sw cells (64/sw strips stacked in k), rows [j0, j1) of
@@ -203,6 +209,17 @@ constexpr int WIN_RING = 4 * NPRIM * 64;
 //
 // LDS of one wave: F[slot][NCONS] doubles, then N[slot][NPRIM] of R,
 // slot = (sub * 2 + side) * ly + r  (side 0: left edge face, 1: right edge face; r = row - j0)
+//
+// Block-cooperative edges (WIDE FP32 strips, MHD_XY_COOP=1): a block's four waves are four
+// neighbouring strips of one x-row, or whole x-rows of one or two strips, of the same row
+// chunk, and march in step.  An edge between two of them needs no pre-pass: each wave
+// publishes its two edge cells as it recovers a row, the right interface state of its last
+// cell and the flux of its first face; two block barriers per row order them.  The wave on
+// the right computes the shared face (lane 0's left face), the wave on the left reads it.
+// With x periodic and the block holding whole x-rows, a row's two end faces pair up the
+// same way (the ghost cells are copies of the cells at the other end).  Only edges without
+// a neighbour in the block use the pre-pass, whose 4-cell neighbourhoods would otherwise be
+// fetched from memory a second time when the march reaches those rows.
 template <typename R>
 __host__ __device__ constexpr size_t xy_slot_bytes() { return NCONS * sizeof(double) + NPRIM * sizeof(R); }
 
@@ -212,13 +229,30 @@ __host__ __device__ inline size_t xy_wave_lds(int sw, int ly) { return (size_t)(
 // WIDE: every strip is one full wave (sw = 64, nx a multiple of 64); x neighbours then move
 // by DPP wave shifts, and the strip-edge values from LDS enter as the shifts' end-lane values.
 //
+// The block-cooperative edges apply to FP32 strips: at two wavefronts per SIMD the other
+// block on the CU runs while one waits at a barrier.  FP64 marches run one wavefront per
+// SIMD and are bound by its instruction issue, where the barriers cost more than the
+// pre-pass they save.
+template <typename R, bool WIDE>
+__host__ __device__ inline bool xy_coop(const FusedArgs& a) {
+  if (!(WIDE && MHD_XY_COOP && sizeof(R) == 4)) return false;
+  if (a.ns % WAVES == 0) return true;
+  // one strip per x-row shares an edge only with itself, across a periodic x
+  if (a.ns == 1 && !(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return false;
+  return WAVES % a.ns == 0 && a.g.nz % (WAVES / a.ns) == 0;
+}
+// static LDS of the edge exchange (FP32 only)
+template <typename R>
+__host__ __device__ constexpr int xy_edge_len(int per_wave) { return sizeof(R) == 4 ? WAVES * per_wave : 1; }
+//
 // FP64 (MHD_XY_RING=1): the window rows and the previous pass's upper y state live in this
 // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold rows j-1, j, j+1 in
 // rotation and slot 3 the upper state of row j-1.  A row is written once per pass and read
 // where it is used instead of being carried (and rotated) in registers, which leaves the
 // registers to the Riemann solves.  FP32 keeps the window in registers.
 template <int EFLUX, typename R, bool WIDE>
-__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds, R* ring) {
+__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds, R* ring, R* edgeP,
+                                        double* edgeF, int wv) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -259,11 +293,27 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
       pin_lanes(1.0 / (a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)]));
   double* ldsF = reinterpret_cast<double*>(lds);
   R* ldsN = reinterpret_cast<R*>(lds + (size_t)SPW * 2 * a.ly * NCONS * sizeof(double));
+  // block-uniform: a block holds 4 neighbouring strips of one x-row (ns a multiple of 4), or
+  // 4/ns whole x-rows of the same y chunk (ns = 1 or 2 and nz a multiple of 4/ns), so all of
+  // its waves march the same rows and take the same barriers
+  const bool coop = xy_coop<R, WIDE>(a);
+  const int rw = min(a.ns, WAVES);  // strips of one x-row in the block
+  const int pos = wv % rw, grp = wv - pos;
+  const bool wrap = coop && a.ns <= WAVES && a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC;
+  const bool coop_l = coop && (pos > 0 || wrap);
+  const bool coop_r = coop && (pos < rw - 1 || wrap);
+  const int nb_l = grp + (pos + rw - 1) % rw, nb_r = grp + (pos + 1) % rw;
+  // edge cell `side` (0 first, 1 last) of row rw published by wave wave
+  auto edge_cell = [&](int wave, int rw, int side) { return edgeP + wave * XY_EDGE_R + ((rw & 3) * 2 + side) * NPRIM; };
+  R* const edge_rt = edgeP + wv * XY_EDGE_R + 8 * NPRIM;
 
-  // pre-pass: the strip-edge x faces of every row of the chunk, one face per lane
-  const int nf = SPW * 2 * nrows;
+  // pre-pass: the strip-edge x faces of every row of the chunk without a block neighbour,
+  // one face per lane
+  const int side0 = coop_l ? 1 : 0;
+  const int nsides = 2 - (coop_l ? 1 : 0) - (coop_r ? 1 : 0);
+  const int nf = SPW * nsides * nrows;
   for (int f = lane; f < nf; f += 64) {
-    const int r = f % nrows, side = (f / nrows) & 1, sb = f / (2 * nrows);
+    const int r = f % nrows, side = side0 + (f / nrows) % nsides, sb = f / (nsides * nrows);
     const int kr = min(NGH + kg * SPW + sb, NGH + g.nz - 1);
     const int fc = side ? ilast + 1 : i0;  // the face between cells fc-1 and fc
     R q[4][NPRIM];
@@ -298,6 +348,14 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   load_prim<R>(U, vs, g.cell(k, j0 - 2, i), a.gam, Pm);
   load_prim<R>(U, vs, g.cell(k, j0 - 1, i), a.gam, P0);
   load_prim<R>(U, vs, g.cell(k, j0, i), a.gam, Pp);
+  if (coop) {
+    if (lane == 0 || lane == 63) {
+      R* e = edge_cell(wv, j0, lane == 63);
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) e[n] = Pp[n];
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
@@ -323,10 +381,12 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
       const int sl = (sub * 2) * a.ly + (row - j0), sr = sl + a.ly;
       if constexpr (PRING) rdr(sm, Pm);
       R qm[NPRIM], qp[NPRIM];
+      const R* srcl = coop_l ? edge_cell(nb_l, row, 1) : ldsN + sl * NPRIM;
+      const R* srcr = coop_r ? edge_cell(nb_r, row, 0) : ldsN + sr * NPRIM;
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) {
-        const R nl = in_reg(ldsN[sl * NPRIM + n]);
-        const R nr = in_reg(ldsN[sr * NPRIM + n]);
+        const R nl = in_reg(srcl[n]);
+        const R nr = in_reg(srcr[n]);
         if constexpr (WIDE) {
           qm[n] = dpp_shift<DPP_WAVE_SHR1>(Pm[n], nl);
           qp[n] = dpp_shift<DPP_WAVE_SHL1>(Pm[n], nr);
@@ -339,22 +399,46 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
       }
       R rt[NPRIM], lt[NPRIM];
       cell_states<R>(qm, Pm, qp, rt, lt);
-      R rtl[NPRIM];
+      if (coop) {
+        if (lane == 63) {
 #pragma unroll
-      for (int n = 0; n < NPRIM; ++n) {
-        if constexpr (WIDE) rtl[n] = dpp_shift<DPP_WAVE_SHR1>(rt[n], rt[n]);  // lane 0's face comes from LDS
-        else rtl[n] = lane_get(rt[n], src_l);
+          for (int n = 0; n < NPRIM; ++n) edge_rt[n] = rt[n];
+        }
+        __syncthreads();
+      }
+      R rtl[NPRIM];
+      if (coop_l) {
+        // lane 0 takes the right state of the left neighbour's last cell
+        const R* ol = edgeP + nb_l * XY_EDGE_R + 8 * NPRIM;
+#pragma unroll
+        for (int n = 0; n < NPRIM; ++n) rtl[n] = dpp_shift<DPP_WAVE_SHR1>(rt[n], in_reg(ol[n]));
+      } else {
+#pragma unroll
+        for (int n = 0; n < NPRIM; ++n) {
+          if constexpr (WIDE) rtl[n] = dpp_shift<DPP_WAVE_SHR1>(rt[n], rt[n]);  // lane 0's face comes from LDS
+          else rtl[n] = lane_get(rt[n], src_l);
+        }
       }
       double Fl[NCONS];
       riemann_flux<0, EFLUX, R>(rtl, lt, chg, hchg, Fl);
+      if (!coop_l) {
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) {
-        const double fl = in_reg(ldsF[sl * NCONS + m]);
-        Fl[m] = is_left ? fl : Fl[m];
+        for (int m = 0; m < NCONS; ++m) {
+          const double fl = in_reg(ldsF[sl * NCONS + m]);
+          Fl[m] = is_left ? fl : Fl[m];
+        }
       }
+      if (coop) {
+        if (lane == 0) {
+#pragma unroll
+          for (int m = 0; m < NCONS; ++m) edgeF[wv * NCONS + m] = Fl[m];
+        }
+        __syncthreads();
+      }
+      const double* srcf = coop_r ? edgeF + nb_r * NCONS : ldsF + sr * NCONS;
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) {
-        const double fe = in_reg(ldsF[sr * NCONS + m]);
+        const double fe = in_reg(srcf[m]);
         if constexpr (WIDE) {
           ax[m] = (dpp_shift<DPP_WAVE_SHL1>(Fl[m], fe) - Fl[m]) * xinv;
         } else {
@@ -383,6 +467,11 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
     R Pn[NPRIM];
     recover_in_place<R>(un, a.gam, Pn);
     if constexpr (PRING) wr(sm, Pn);
+    if (coop && (lane == 0 || lane == 63)) {
+      R* e = edge_cell(wv, j + 2, lane == 63);
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) e[n] = Pn[n];
+    }
     if (out_row && active) {
       double d[NCONS];
 #pragma unroll
@@ -420,9 +509,11 @@ template <int EFLUX, typename R, bool WIDE>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ R xring[xy_ring_len<R>()];
+  __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
+  __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int wv = threadIdx.x >> 6;
   xy_body<EFLUX, R, WIDE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly),
-                          xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0));
+                          xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0), xedgeP, xedgeF, wv);
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
@@ -626,13 +717,15 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
                                                                      const int2* __restrict__ tab) {
   extern __shared__ __align__(16) char smem[];
   __shared__ R xring[xy_ring_len<R>()];
+  __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
+  __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
   const int wv = threadIdx.x >> 6;
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
   R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
-  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring);
-  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring);
+  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
 }
 
 template <int EFLUX, typename R>
@@ -711,9 +804,16 @@ void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
 
 }  // namespace
 
+size_t fused_xy_lds_max(int mixed) {
+  if (mixed) return FUSED_XY_LDS_MAX;
+  const size_t fixed = sizeof(double) * ((size_t)xy_ring_len<double>() + xy_edge_len<double>(XY_EDGE_R) +
+                                         xy_edge_len<double>(NCONS));
+  return std::min(FUSED_XY_LDS_MAX, XY_LDS_PER_CU - fixed);
+}
+
 int fused_max_ly(int sw, int mixed) {
   const size_t per_row = (size_t)WAVES * (mixed ? xy_wave_lds<float>(sw, 1) : xy_wave_lds<double>(sw, 1));
-  return (int)(FUSED_XY_LDS_MAX / per_row);
+  return (int)(fused_xy_lds_max(mixed) / per_row);
 }
 
 size_t fused_xy_lds(const FusedArgs& a) {

@@ -39,10 +39,14 @@ struct FusedArgs {
 // address columns with 32-bit offsets)
 bool fused_configure(FusedArgs& a, bool alone = true);
 
-// LDS of the xy kernel: per block (WAVES = 4 waves) for a box's tiling, and the longest
-// y chunk a strip width allows within FUSED_XY_LDS_MAX bytes per block
+// Dynamic LDS of the xy kernel (the pre-pass slots): per block (WAVES = 4 waves) for a box's
+// tiling, the most a block may take (FUSED_XY_LDS_MAX, or for FP64 what the static window
+// rings and edge exchange leave of the CU's XY_LDS_PER_CU bytes), and the longest y chunk a
+// strip width allows within it
 constexpr size_t FUSED_XY_LDS_MAX = 64 * 1024;
+constexpr size_t XY_LDS_PER_CU = 160 * 1024;
 size_t fused_xy_lds(const FusedArgs& a);
+size_t fused_xy_lds_max(int mixed);
 int fused_max_ly(int sw, int mixed);
 
 // blocks of 256 threads each kernel needs for one box

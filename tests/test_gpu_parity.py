@@ -175,6 +175,44 @@ def test_small_and_ragged_boxes(orc, shape):
         assert np.all(errU <= TOL_L1), (mode, errU)
 
 
+@pytest.mark.parametrize("shape,bc", [
+    ((256, 10, 8), (1, 1, 2, 2, 1, 1)),   # four 64-cell strips, one block per row, x ends paired (periodic)
+    ((256, 10, 8), (3, 3, 2, 2, 1, 1)),   # same blocks, x ends from the pre-pass (outflow)
+    ((512, 6, 5), (1, 1, 2, 2, 1, 1)),    # two blocks per row: the edge between them from the pre-pass
+    ((128, 8, 10), (1, 1, 2, 2, 1, 1)),   # two strips per row, two rows per block
+    ((128, 8, 9), (1, 1, 2, 2, 1, 1)),    # odd row count: rows of a block would straddle y chunks, no sharing
+    ((64, 6, 8), (1, 1, 2, 2, 1, 1)),     # one strip per row, four rows per block, periodic ends
+    ((64, 6, 8), (3, 3, 2, 2, 1, 1)),     # one strip per row, outflow: nothing to share
+])
+def test_block_cooperative_strip_edges(orc, shape, bc):
+    """x-rows of whole-wave strips whose block neighbours exchange edge cells, interface
+    states and face fluxes through LDS (FP32 strips): FP64 against the oracle, and the mixed
+    path against the oracle and against the same box run with 32-cell strips (no sharing)"""
+    say("block-shared strip edges", shape, bc)
+    kw = dict(bc=bc, energy_flux=1, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), grid_shift=0)
+    ic = dict(bx0=0.1, a=0.06, amp=0.04)
+    errP, errU, _, _ = _run_parity(orc, shape, 12, 0, problem=ic, **kw)
+    assert np.all(errP <= TOL_L1), errP
+    assert np.all(errU <= TOL_L1), errU
+    mixed = []
+    for sw in (64, 32):
+        o, b = pair(orc, n=shape, problem=ic, precision=1, **kw)
+        if sw == 32:
+            _, ly, lz = b.get_tiling()
+            b.set_tiling(32, min(ly, 32), lz)
+        o.run(12)
+        b.run(12, mode=0)
+        P = b.download_prim()
+        err = rel_l1(o.interior(P), o.interior(o.P))
+        say("mixed sw", sw, "relL1", err)
+        assert err[0] < 1e-6 and err[5] < 1e-6
+        mixed.append(o.interior(P))
+    # the two runs compute the strip-edge faces in different code (march vs pre-pass), where
+    # the compiler may contract FP32 products differently: ulp-level flux differences that
+    # stay near 1e-10 after 12 steps; a wrong neighbour cell would show at 1e-6 or above
+    assert rel_l1(mixed[0], mixed[1]).max() < 1e-8
+
+
 def test_tmax_stop_matches_reference_loop(orc):
     o, b = pair(orc, n=(16, 16, 16))
     dt0 = o.control_timestep()

@@ -76,6 +76,8 @@ def test_tiling_limits_refused():
         b.set_tiling(64, 8, 65)       # z chunk above 64 planes
     with pytest.raises(MhdError):
         b.set_tiling(64, 65, 8)       # y chunk above ny
+    with pytest.raises(MhdError):
+        b.set_tiling(64, 47, 8)       # one row beyond the FP64 pre-pass slots at sw = 64
     b.set_tiling(64, 46, 64)          # the largest y chunk FP64 allows at sw = 64
     assert b.get_tiling() == (64, 46, 64)
     b.close()
