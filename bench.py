@@ -37,10 +37,14 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on MI355X, each
 # counter scaled by what it reads for a device copy of known size (fetch x2.0, write x1.0);
-# summary and commands in profiles/fused_fp64_256cube_pmc.md.  Keyed by
-# (n, mode, precision, kernel); a workload without a measurement reports null.
+# summaries and commands in profiles/fused_fp64_256cube_pmc.md and
+# profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
+# without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1324557.6 * 1024 * 2.0 + 1310721.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1347465.9 * 1024 * 2.0 + 1310721.1 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373529.0 * 1024 * 2.0 + 1392896.9 * 1024 * 1.0,
+    (256, 0, 1, "fused_xy"): 757952.3 * 1024 * 2.0 + 1310812.7 * 1024 * 1.0,
+    (256, 0, 1, "fused_z_update"): 1382737.9 * 1024 * 2.0 + 1393397.6 * 1024 * 1.0,
 }
 
 
