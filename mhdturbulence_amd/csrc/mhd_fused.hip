@@ -146,6 +146,21 @@ __device__ __forceinline__ float dpp_shift(float v, float old) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
+// x neighbour values in a wavefront of strips: lane l takes lane l-1's (from_left) or l+1's
+// (from_right) value.  Strip-edge lanes get a neighbour strip's value or their own and are
+// overwritten by the caller's strip-edge select.  MHD_XY_DPP: one DPP wave shift per dword;
+// otherwise a ds_bpermute from the clamped in-strip source lane.
+template <typename R>
+__device__ __forceinline__ R from_left(R v, int src_l) {
+  if constexpr (MHD_XY_DPP) return dpp_shift<DPP_WAVE_SHR1>(v, v);
+  else return lane_get(v, src_l);
+}
+template <typename R>
+__device__ __forceinline__ R from_right(R v, int src_r) {
+  if constexpr (MHD_XY_DPP) return dpp_shift<DPP_WAVE_SHL1>(v, v);
+  else return lane_get(v, src_r);
+}
+
 // a value the compiler must hold in registers as is: an LDS read passed through here stays
 // an unconditional read feeding a lane select, instead of becoming a branch per value
 template <typename T>
@@ -391,8 +406,8 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
           qm[n] = dpp_shift<DPP_WAVE_SHR1>(Pm[n], nl);
           qp[n] = dpp_shift<DPP_WAVE_SHL1>(Pm[n], nr);
         } else {
-          const R vm = lane_get(Pm[n], src_l);
-          const R vp = lane_get(Pm[n], src_r);
+          const R vm = from_left(Pm[n], src_l);
+          const R vp = from_right(Pm[n], src_r);
           qm[n] = is_left ? nl : vm;
           qp[n] = is_right ? nr : vp;
         }
@@ -416,7 +431,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
 #pragma unroll
         for (int n = 0; n < NPRIM; ++n) {
           if constexpr (WIDE) rtl[n] = dpp_shift<DPP_WAVE_SHR1>(rt[n], rt[n]);  // lane 0's face comes from LDS
-          else rtl[n] = lane_get(rt[n], src_l);
+          else rtl[n] = from_left(rt[n], src_l);
         }
       }
       double Fl[NCONS];
@@ -442,7 +457,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
         if constexpr (WIDE) {
           ax[m] = (dpp_shift<DPP_WAVE_SHL1>(Fl[m], fe) - Fl[m]) * xinv;
         } else {
-          const double fr = lane_get(Fl[m], src_r);
+          const double fr = from_right(Fl[m], src_r);
           ax[m] = ((is_right ? fe : fr) - Fl[m]) * xinv;
         }
       }

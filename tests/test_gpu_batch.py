@@ -117,3 +117,29 @@ def test_batch_mixed_precision_close_to_fp64():
     err = parity.rel_l1(Pm[s].transpose(2, 0, 1, 3), Pd[s].transpose(2, 0, 1, 3), "prim")
     say("mixed vs fp64", err)
     assert err[[0, 5]].max() < 1e-6 and err.max() < 1e-4
+
+
+def test_batch_mixed_shared_strip_edges_alone_equals_mix(orc):
+    """mixed precision in a batch with boxes whose FP32 strips share their edge faces in the
+    block (one, two and four 64-cell strips per x-row) beside a narrow-strip box: each box
+    is bitwise the same alone and in the mix, and close to the FP64 oracle"""
+    shapes = [(64, 16, 24), (128, 8, 10), (256, 6, 4), (48, 32, 16)]
+    mix = _batch(precision=1)
+    ids = [mix.add_box(*s, ic=ic) for s, ic in zip(shapes, ICS)]
+    mix.step(20)
+    for bid, s, ic in zip(ids, shapes, ICS):
+        _, Pm = mix.finish_box(bid, want_prim=True)
+        alone = _batch(precision=1)
+        a = alone.add_box(*s, ic=ic)
+        alone.step(20)
+        _, Pa = alone.finish_box(a, want_prim=True)
+        alone.close()
+        np.testing.assert_array_equal(Pa, Pm)
+        o = orc.Oracle(nx=s[0], ny=s[1], nz=s[2], **BASE)
+        o.generate_grid()
+        o.generate_problem_shear(**ic)
+        o.run(20)
+        err = parity.rel_l1(o.interior(Pm), o.interior(o.P), "prim")
+        say(s, "mixed vs oracle", err[[0, 5]])
+        assert err[[0, 5]].max() < 1e-6 and err.max() < 1e-4, (s, err)
+    mix.close()
