@@ -142,6 +142,35 @@ def test_kh_64cube_100_steps(orc, mode):
     assert np.all(errU <= TOL_L1), errU
 
 
+def test_kh_150cube_step_100_checkpoint(orc):
+    """the reference KH problem at 150^3 (its README benchmark shape, 32-cell strips with
+    ragged edges), fused and staged paths against the oracle at step 100"""
+    n = (150, 150, 150)
+    o = orc.Oracle(nx=n[0], ny=n[1], nz=n[2])
+    o.generate_grid()
+    o.generate_problem_kh()
+    boxes = []
+    for mode in (0, 1):
+        b = Box(Config(nx=n[0], ny=n[1], nz=n[2]))
+        b.set_grid(o.grid(1), o.grid(1, True), o.grid(2), o.grid(2, True), o.grid(3), o.grid(3, True))
+        b.upload_prim(o.P)
+        b.upload_cons(o.U)
+        boxes.append((mode, b))
+    say("KH 150^3: oracle 100 steps")
+    t_o, log_o = o.run(100)
+    for mode, b in boxes:
+        t_g, log_g, done = b.run(100, mode=mode)
+        assert done == 100
+        np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+        assert t_g == pytest.approx(t_o, rel=1e-12)
+        errP = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+        errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+        say(f"150^3 mode={mode} max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+        assert np.all(errP <= TOL_L1), (mode, errP)
+        assert np.all(errU <= TOL_L1), (mode, errU)
+        b.close()
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("eflux", [0, 1])
 def test_magnetised_anisotropic(orc, mode, eflux):
