@@ -22,7 +22,7 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiling
 
 def candidates(nx: int, ny: int, nz: int):
     sws = [sw for sw in (64, 32, 16) if sw <= max(16, nx)]
-    lys = sorted({min(ny, v) for v in (8, 16, 32, 64)})
+    lys = sorted({min(ny, v) for v in (8, 16, 24, 32, 40, 46, 48, 64)})
     lzs = sorted({min(nz, v) for v in (8, 16, 32, 64)})
     return list(itertools.product(sws, lys, lzs))
 

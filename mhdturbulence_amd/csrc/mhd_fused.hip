@@ -844,16 +844,20 @@ bool fused_configure(FusedArgs& a, bool alone) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
-  // y chunk: 32 rows (fewer re-read halo rows and pre-passes); a box run alone that is too
-  // small to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs) goes
-  // down to 8 rows (in a batch the other boxes fill the chip)
+  // y chunk: a box run alone takes up to 48 rows (fewer re-read halo rows and pre-passes),
+  // as far as the xy kernel's LDS allows, and goes down towards 8 rows while it is too small
+  // to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs); in a batch
+  // (the other boxes fill the chip) 32 rows
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
-  a.ly = std::min({32, a.g.ny, fused_max_ly(best, a.mixed)});
-  while (alone && a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly /= 2;
-  // z chunk: the longest (fewest re-read halo planes, up to 64) that, for a box run alone,
-  // still gives the z kernel two wavefronts per SIMD of the chip; 32 planes in a batch
+  a.ly = std::min({alone ? 48 : 32, a.g.ny, fused_max_ly(best, a.mixed)});
+  while (alone && a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly = std::max(8, a.ly / 2);
+  // z chunk: for a box run alone the longest that still gives the z kernel two wavefronts
+  // per SIMD of the chip, up to 64 planes in FP64 (fewest re-read halo planes; one
+  // wavefront per SIMD runs at a time) and 16 in mixed precision (two per SIMD: its
+  // HBM-bound z kernel gains more from wavefronts in flight than it loses to halo planes);
+  // 32 planes in a batch
   const long long cols = (long long)a.ns * ((a.g.ny + 64 / best - 1) / (64 / best));
-  a.lz = std::min(alone ? 64 : 32, a.g.nz);
+  a.lz = std::min(alone ? (a.mixed ? 16 : 64) : 32, a.g.nz);
   while (alone && a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
