@@ -583,9 +583,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1
   // in rotation (sm, s0, sp) and slot 3 the upper state of plane k-1.  A plane is written
   // once and read once per pass instead of being copied through registers, which leaves the
-  // registers to the Riemann solve.  FP32 (two wavefronts per SIMD, half the LDS each) keeps
-  // them in registers (wm, w0, wp, wu).  In both, raw[((p % 3) * NCONS + m) * 64 + lane]
-  // holds the conserved state of plane p for the last three planes loaded.
+  // registers to the Riemann solve.  FP32 (two blocks per CU, which leaves no LDS for a
+  // window ring) keeps them in registers (wm, w0, wp, wu).  In both,
+  // raw[((p % 3) * NCONS + m) * 64 + lane] holds the conserved state of plane p for the last
+  // three planes loaded.
   constexpr bool PRING = sizeof(R) == 8;
   R wm[NPRIM], w0[NPRIM], wp[NPRIM], wu[NPRIM];
   auto rdr = [&](int slot, R* q) {

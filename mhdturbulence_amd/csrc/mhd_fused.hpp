@@ -2,8 +2,9 @@
 //
 //   xy   : x- and y-face fluxes and their divergence  acc = dFx/dx + dFy/dy
 //          (a wavefront holds a 64/32/16-cell x-strip in its lanes; x neighbours come
-//          through cross-lane shuffles, the strip-edge faces from a per-chunk pre-pass
-//          parked in LDS; it marches in y keeping a 3-row register window)
+//          through DPP wave shifts, the strip-edge faces from a per-chunk pre-pass parked
+//          in LDS or, for FP32 full-wave strips, from the neighbouring wavefronts of the
+//          block; it marches in y keeping a 3-row window, in LDS for FP64)
 //   z    : z-face fluxes marching in z, then U_new = U - dt*(acc + dFz/dz), psi damping,
 //          primitive recovery of U_new, the next iteration's dt / c_h reduction, and the
 //          ghost cells of U_new (periodic / reflecting / outflow images)
