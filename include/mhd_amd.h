@@ -88,6 +88,15 @@ int mhd_download_prim(mhd_ctx* ctx, double* P);
 int mhd_download_cons(mhd_ctx* ctx, double* U);
 int mhd_download_flux(mhd_ctx* ctx, int dir, double* F);
 
+/* Asynchronous snapshot of P (no counterpart in the reference, whose Output reads the host
+ * mirror synchronously, src_cpp_kokkos_host/output.cpp:262-280): _begin queues a copy of the
+ * current primitive state, in the host layout above, into library-owned pinned host memory
+ * on a second stream and returns at once, so the transfer overlaps the steps that follow;
+ * _end waits for it and points *P_host at the copy, which stays valid until the next _begin
+ * or mhd_destroy.  A _begin while an earlier snapshot is in flight first waits for it. */
+int mhd_snapshot_begin(mhd_ctx* ctx);
+int mhd_snapshot_end(mhd_ctx* ctx, const double** P_host);
+
 /* ---- the reference step functions, one call each ---- */
 /* ControlTimestep (src_cpp_kokkos_host/mhd.cpp:1511-1561): *dt = cfl*min CFL bound of P */
 int mhd_control_timestep(mhd_ctx* ctx, double* dt);

@@ -16,7 +16,7 @@ EXPORTS = (
     "mhd_version", "mhd_error_string", "mhd_last_error", "mhd_create", "mhd_destroy", "mhd_get_dims",
     "mhd_device_bytes", "mhd_set_grid", "mhd_generate_grid", "mhd_get_grid", "mhd_upload_prim",
     "mhd_upload_cons", "mhd_download_prim", "mhd_download_cons", "mhd_download_flux",
-    "mhd_control_timestep", "mhd_set_boundary", "mhd_evaluate_ch", "mhd_numerical_flux",
+    "mhd_snapshot_begin", "mhd_snapshot_end", "mhd_control_timestep", "mhd_set_boundary", "mhd_evaluate_ch", "mhd_numerical_flux",
     "mhd_update_conserv", "mhd_damp_psi", "mhd_update_primitive", "mhd_run", "mhd_generate_problem_kh",
     "mhd_generate_problem_shear", "mhd_diagnostics", "mhd_status", "mhd_last_run_timing",
     "mhd_profile_kernels", "mhd_kernel_names",
@@ -80,6 +80,8 @@ def load(path: str | None = None):
         "mhd_download_prim": (C.c_int, [vp, dp]),
         "mhd_download_cons": (C.c_int, [vp, dp]),
         "mhd_download_flux": (C.c_int, [vp, C.c_int, dp]),
+        "mhd_snapshot_begin": (C.c_int, [vp]),
+        "mhd_snapshot_end": (C.c_int, [vp, C.POINTER(dp)]),
         "mhd_control_timestep": (C.c_int, [vp, dp]),
         "mhd_set_boundary": (C.c_int, [vp]),
         "mhd_evaluate_ch": (C.c_int, [vp, dp]),

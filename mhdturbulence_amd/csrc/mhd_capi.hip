@@ -77,6 +77,13 @@ struct mhd_ctx {
   double* acc = nullptr;
   mhd::FusedArgs fa{};
   bool fa_ready = false;
+  // asynchronous snapshot: P in host layout on the device, its pinned host copy, the copy
+  // stream and the events ordering it after the step stream
+  hipStream_t copy_stream = nullptr;
+  double* snap_dev = nullptr;
+  double* snap_host = nullptr;
+  hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+  bool snap_recorded = false, snap_pending = false;
 };
 
 namespace {
@@ -193,14 +200,19 @@ int mhd_destroy(mhd_ctx* c) {
   // teardown: errors are not actionable here, every release is attempted
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   drop_graph(c);
   double* bufs[] = {c->P, c->U, c->F[0], c->F[1], c->F[2], c->U2, c->acc, c->staging, c->diag, c->dt_log,
-                    c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5]};
+                    c->grid[0], c->grid[1], c->grid[2], c->grid[3], c->grid[4], c->grid[5], c->snap_dev};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (c->sc) (void)hipFree(c->sc);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->snap_host) (void)hipHostFree(c->snap_host);
+  if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+  if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MHD_OK;
@@ -320,6 +332,47 @@ int mhd_download_cons(mhd_ctx* c, double* U) { return download(c, mhd::NCONS, c 
 int mhd_download_flux(mhd_ctx* c, int dir, double* F) {
   if (!c || dir < 1 || dir > 3) return fail(MHD_ERR_ARG, "dir is 1, 2 or 3");
   return download(c, mhd::NCONS, c->F[dir - 1], F);
+}
+
+int mhd_snapshot_begin(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  CHKRET(set_dev(c));
+  const size_t n = mhd::NPRIM * host_cells(c);
+  if (!c->copy_stream) {
+    // default priority, as the step stream
+    HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+  }
+  CHKRET(dalloc(c, &c->snap_dev, n));
+  if (!c->snap_host) {
+    hipError_t e = hipHostMalloc((void**)&c->snap_host, n * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->snap_host = nullptr;
+      return fail(MHD_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+  }
+  // the previous snapshot's transfer still reads snap_dev and writes snap_host
+  if (c->snap_recorded) HIPCHK(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+  mhd::launch_to_host_layout(c->g, mhd::NPRIM, c->P, c->snap_dev, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->snap_ready, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+  HIPCHK(hipMemcpyAsync(c->snap_host, c->snap_dev, n * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+  HIPCHK(hipEventRecord(c->snap_done, c->copy_stream));
+  c->snap_recorded = true;
+  c->snap_pending = true;
+  return MHD_OK;
+}
+
+int mhd_snapshot_end(mhd_ctx* c, const double** P_host) {
+  if (!c || !P_host) return fail(MHD_ERR_ARG, "null argument");
+  if (!c->snap_pending) return fail(MHD_ERR_STATE, "no snapshot begun (mhd_snapshot_begin first)");
+  CHKRET(set_dev(c));
+  HIPCHK(hipEventSynchronize(c->snap_done));
+  c->snap_pending = false;
+  *P_host = c->snap_host;
+  return MHD_OK;
 }
 
 static int need_grid(mhd_ctx* c) {

@@ -8,6 +8,10 @@ Files for snapshot index `nout` under `outdir`:
   bindata/field%05d.bin    [nvars][z][y][x] f64, d v1 v2 v3 b1 b2 b3 bp p X1 (output.cpp:172-203, 262-280)
   bindata/field%05d.xmf    XDMF 2 3DRectMesh pointing into the two above (output.cpp:285-360)
   ascdata/snap000-%05d.csv x y d vx vy p X1 on the first interior z plane (output.cpp:365-407)
+
+The state can come from Box.download_prim() or, without stalling the run, from
+Box.snapshot_begin() ... Box.snapshot_end(): the copy to pinned host memory then travels on a
+second stream while the following steps run.
 """
 from __future__ import annotations
 

@@ -120,6 +120,19 @@ class Box:
         capi.check(self.L.mhd_download_prim(self.h, _dp(P)))
         return P
 
+    def snapshot_begin(self):
+        """queue a copy of the current primitive state to host memory; later steps overlap it"""
+        capi.check(self.L.mhd_snapshot_begin(self.h))
+
+    def snapshot_end(self, copy: bool = True) -> np.ndarray:
+        """wait for the snapshot begun last; copy=False returns a view of the library's pinned
+        buffer, valid until the next snapshot_begin or close"""
+        ptr = C.POINTER(C.c_double)()
+        capi.check(self.L.mhd_snapshot_end(self.h, C.byref(ptr)))
+        shape = self._shape(NPRIM)
+        view = np.ctypeslib.as_array(ptr, shape=(int(np.prod(shape)),)).reshape(shape)
+        return view.copy() if copy else view
+
     def download_cons(self) -> np.ndarray:
         U = np.zeros(self._shape(NCONS))
         capi.check(self.L.mhd_download_cons(self.h, _dp(U)))
