@@ -52,9 +52,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_box(n: int, mode_precision: int):
+def make_box(shape, mode_precision: int):
     from mhdturbulence_amd import Box, Config
-    cfg = Config(nx=n, ny=n, nz=n, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+    nx, ny, nz = shape
+    cfg = Config(nx=nx, ny=ny, nz=nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
                  gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0, precision=mode_precision)
     b = Box(cfg)
     b.generate_grid()
@@ -62,12 +63,13 @@ def make_box(n: int, mode_precision: int):
     return b
 
 
-def cpu_baseline(n: int, steps: int) -> dict:
+def cpu_baseline(shape, steps: int) -> dict:
     """oracle on this host's cores, same box, `steps` steps (after building its state)"""
     from oracle import oracle as orc
+    nx, ny, nz = shape
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     orc.lib().orc_set_threads(threads)
-    o = orc.Oracle(nx=n, ny=n, nz=n, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+    o = orc.Oracle(nx=nx, ny=ny, nz=nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
                    gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0)
     o.generate_grid()
     o.generate_problem_shear(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1, seed=20261203)
@@ -75,8 +77,9 @@ def cpu_baseline(n: int, steps: int) -> dict:
     t0 = time.perf_counter()
     o.run(steps)
     wall = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same {n}^3 box (oracle C++/OpenMP restatement), {wall:.1f} s"}
+    name = f"{nx}^3" if nx == ny == nz else f"{nx}x{ny}x{nz}"
+    return {"value": nx * ny * nz * steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {name} box (oracle C++/OpenMP restatement), {wall:.1f} s"}
 
 
 SHEAR_BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
@@ -170,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--shape", type=int, nargs=3, metavar=("NX", "NY", "NZ"), default=None,
+                    help="a non-cubic box instead of n^3 (e.g. one GPU's z-slab of a sharded box)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
@@ -177,8 +182,12 @@ def main():
         sys.exit(2)
 
     n = args.n
-    log(f"[bench] building {n}^3 box on the device")
-    box = make_box(n, args.precision)
+    shape = tuple(args.shape) if args.shape else (n, n, n)
+    cells = shape[0] * shape[1] * shape[2]
+    cube = shape[0] == shape[1] == shape[2]
+    shape_name = f"{n}^3" if cube and not args.shape else "x".join(map(str, shape))
+    log(f"[bench] building {shape_name} box on the device")
+    box = make_box(shape, args.precision)
     log(f"[bench] device bytes {box.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
     t = 0.0
     t, _, _ = box.run(args.warmup, time=t, mode=args.mode)
@@ -187,7 +196,7 @@ def main():
     prof = box.profile_kernels(args.steps, mode=args.mode)  # returns after the stream drained
     wall = time.perf_counter() - t0
     status = box.status()
-    value = n ** 3 * args.steps / wall
+    value = cells * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     log(f"[bench] {value:.4e} cell-updates/s, {ms_per_step:.3f} ms/step (kernels {sum(prof.values()):.3f} ms/step)")
     # the production loop (each step a replayed graph, no events) over --profile-steps steps
@@ -201,11 +210,12 @@ def main():
     # algorithmic bytes of one launch of the dominant kernel: the per-cell-update figure (160 B,
     # read + write of the FP64 conserved state) times the cells that launch processes (every
     # interior cell); "step" relates the same bytes to the whole step's kernel time
-    alg_bytes = BYTES_PER_CELL * n ** 3
+    alg_bytes = BYTES_PER_CELL * cells
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": MEASURED_TRAFFIC.get((n, args.mode, args.precision, dominant)), "kernel": dominant,
+                "traffic": MEASURED_TRAFFIC.get((shape[0], args.mode, args.precision, dominant)) if cube else None,
+                "kernel": dominant,
                 "kernel_ms": round(dom_ms, 4),
                 "step": {"achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2),
                          "frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
@@ -215,8 +225,8 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f64 state, f32 reconstruction+HLLD",
         "data": "synthetic (device-generated seeded shear-layer state)",
-        "config": {"workload": f"KH shear layer {n}^3, {'fused' if args.mode == 0 else 'staged'} step kernels",
-                   "cells": n ** 3, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": 0.05, "stages": 1,
+        "config": {"workload": f"KH shear layer {shape_name}, {'fused' if args.mode == 0 else 'staged'} step kernels",
+                   "cells": cells, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": 0.05, "stages": 1,
                    "energy_flux": "(v.B)B_n", "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
@@ -231,8 +241,8 @@ def main():
         log(f"[bench] serving: {args.serve_requests} requests, batched vs one at a time")
         out["serving"] = serving_bench(args.serve_requests)
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {n}^3")
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {shape_name}")
+        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -845,21 +845,25 @@ bool fused_configure(FusedArgs& a, bool alone) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
-  // y chunk: a box run alone takes up to 48 rows (fewer re-read halo rows and pre-passes),
-  // as far as the xy kernel's LDS allows, and goes down towards 8 rows while it is too small
-  // to give the xy kernel a wavefront per SIMD of the chip (256 CUs x 4 SIMDs); in a batch
-  // (the other boxes fill the chip) 32 rows
+  // Chunk lengths.  A box run alone splits ny (nz) into the fewest chunks of at most 48 rows
+  // (64 planes in FP64, 16 in mixed precision) that still give the xy kernel as many
+  // wavefronts as the chip's SIMDs (256 CUs x 4) hold at once (one per SIMD in FP64, two in
+  // FP32) and the z kernel two per SIMD, as even as the split allows and no shorter than 8:
+  // fewer chunks re-read fewer halo rows (planes) and run fewer pre-passes.  The mixed z kernel is HBM-bound at two
+  // wavefronts per SIMD and gains more from wavefronts in flight than it loses to halo
+  // planes, hence its short cap.  In a batch (the other boxes fill the chip) 32 rows and 32
+  // planes.
+  auto split = [](int n, int cap, long long per_chunk, long long want) {
+    int c = (n + cap - 1) / cap;
+    while ((n + c - 1) / c > 8 && per_chunk * c < want) ++c;
+    return std::max(std::min(n, 8), (n + c - 1) / c);
+  };
+  const int ycap = std::min(alone ? 48 : 32, fused_max_ly(best, a.mixed));
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
-  a.ly = std::min({alone ? 48 : 32, a.g.ny, fused_max_ly(best, a.mixed)});
-  while (alone && a.ly > 8 && strips * ((a.g.ny + a.ly - 1) / a.ly) < 1024) a.ly = std::max(8, a.ly / 2);
-  // z chunk: for a box run alone the longest that still gives the z kernel two wavefronts
-  // per SIMD of the chip, up to 64 planes in FP64 (fewest re-read halo planes; one
-  // wavefront per SIMD runs at a time) and 16 in mixed precision (two per SIMD: its
-  // HBM-bound z kernel gains more from wavefronts in flight than it loses to halo planes);
-  // 32 planes in a batch
+  a.ly = alone ? split(a.g.ny, ycap, strips, a.mixed ? 2048 : 1024) : std::min(ycap, a.g.ny);
   const long long cols = (long long)a.ns * ((a.g.ny + 64 / best - 1) / (64 / best));
-  a.lz = std::min(alone ? (a.mixed ? 16 : 64) : 32, a.g.nz);
-  while (alone && a.lz > 8 && cols * ((a.g.nz + a.lz - 1) / a.lz) < 2048) a.lz /= 2;
+  const int zcap = alone ? (a.mixed ? 16 : 64) : 32;
+  a.lz = alone ? split(a.g.nz, zcap, cols, 2048) : std::min(zcap, a.g.nz);
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
 
