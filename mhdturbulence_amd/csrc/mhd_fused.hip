@@ -53,6 +53,27 @@ __device__ __forceinline__ void stc(double* __restrict__ A, size_t vs, int m, un
   *(__attribute__((address_space(1))) double*)(sbase(A + (size_t)m * vs) + ob) = v;
 }
 
+// The NCONS column bases of one array, made scalar once before a march: readfirstlane is
+// convergent, so the compiler does not hoist it out of a loop by itself.
+struct Cols {
+  gchar* p[NCONS];
+};
+__device__ __forceinline__ Cols make_cols(const double* A, size_t vs) {
+  Cols c;
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) c.p[m] = sbase(A + (size_t)m * vs);
+  return c;
+}
+__device__ __forceinline__ void load_raw(const Cols& U, int c, double* u) {
+  const unsigned ob = boff(c);
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) u[m] = *(const __attribute__((address_space(1))) double*)(U.p[m] + ob);
+}
+__device__ __forceinline__ void store_cols(const Cols& A, unsigned ob, const double* u) {
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) *(__attribute__((address_space(1))) double*)(A.p[m] + ob) = u[m];
+}
+
 // The flux divergence acc passed from the xy to the z kernel is cell-major, NCONS doubles
 // per cell at acc + c * NCONS: one 64-bit address per lane and five 16-byte accesses with
 // immediate offsets (rows of a wavefront still cover one contiguous span).
@@ -363,6 +384,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   load_prim<R>(U, vs, g.cell(k, j0 - 2, i), a.gam, Pm);
   load_prim<R>(U, vs, g.cell(k, j0 - 1, i), a.gam, P0);
   load_prim<R>(U, vs, g.cell(k, j0, i), a.gam, Pp);
+  const Cols Uc = make_cols(U, vs);
   if (coop) {
     if (lane == 0 || lane == 63) {
       R* e = edge_cell(wv, j0, lane == 63);
@@ -388,7 +410,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   // latency; only computed values cross the loop edge, so no load in flight is copied.
   for (int j = j0 - 1; j <= j1; ++j) {
     double un[NCONS];
-    load_raw(U, vs, g.cell(k, j + 2, i), un);  // row j+2 <= ny+4 < jtot
+    load_raw(Uc, g.cell(k, j + 2, i), un);  // row j+2 <= ny+4 < jtot
     const int row = j - 1;  // the cell row completed in this pass
     const bool out_row = row >= j0;
     double ax[NCONS];
@@ -599,6 +621,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   };
   load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, wm);
   load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, w0);
+  const Cols Uc = make_cols(U, vs), Uoc = make_cols(Uo, vs);
   {
     double u0[NCONS];
     load_raw(U, vs, g.cell(k0, j, i), u0);
@@ -627,7 +650,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     const int kk = k - 1;  // the cell updated in this pass
     const bool upd = kk >= k0 && active;
     double un[NCONS];
-    load_raw(U, vs, g.cell(k + 2, j, i), un);  // plane k+2 <= nz+4 < ktot
+    load_raw(Uc, g.cell(k + 2, j, i), un);  // plane k+2 <= nz+4 < ktot
     double uo[NCONS], ac[NCONS];
     const unsigned oc = boff(g.cell(kk, j, i));
     const double dza = lane_row(dz_l, max(kk - k0, 0));
@@ -681,8 +704,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
       const double dhl = fmin(dxa, fmin(dya, dza));
       const double taui = a.alphabp * chg / dhl;
       u[mbps] = u[mbps] * (1.0 - dt * taui);
-#pragma unroll
-      for (int m = 0; m < NCONS; ++m) stc(Uo, vs, m, oc, u[m]);
+      store_cols(Uoc, oc, u);
       if (img_xy || kk <= NGH + 1 || kk >= NGH + g.nz - 2) write_images(Uo, g, a.bc, i, j, kk, u);
       // the next iteration's ControlTimestep / EvaluateCh bounds of this cell
       double q[NPRIM];
