@@ -88,6 +88,19 @@ template <>
 __device__ __forceinline__ double sq_pos<double>(double x) {
   return sq_rsq(x);
 }
+// sqrt(x) and 1/sqrt(x) of a positive x from one reciprocal square root: the Goldschmidt
+// pair (g, h) -> (sqrt x, 1/(2 sqrt x)) takes a second step for h, so b / sqrt(x) becomes
+// the product b * (2h) instead of a division (both within about an ulp)
+__device__ __forceinline__ void sq_rsq_pos(double x, double& s, double& rs) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-h, g, 0.5);
+  s = fma(fma(-g, g, x), h, g);
+  rs = 2.0 * fma(h, r, h);
+}
 template <>
 __device__ __forceinline__ float sq<float>(float x) {
   return __builtin_amdgcn_sqrtf(x);
@@ -97,6 +110,16 @@ __device__ __forceinline__ float sq_pos<float>(float x) {
   return __builtin_amdgcn_sqrtf(x);
 }
 #endif
+#if !MHD_FAST_SQRT
+__device__ __forceinline__ void sq_rsq_pos(double x, double& s, double& rs) {
+  s = sqrt(x);
+  rs = 1.0 / s;
+}
+#endif
+__device__ __forceinline__ void sq_rsq_pos(float x, float& s, float& rs) {
+  s = sq_pos(x);
+  rs = dv(1.0f, s);
+}
 
 // Half the van Leer harmonic limiter of KH/mhd.cpp:89-95: vL(dp, dm) / 2 = dp dm / (dp + dm)
 // where dp dm > 0, else 0.  Scaling by 2 is exact in binary floating point (away from the
@@ -255,8 +278,9 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
                   mins1 * eer;
 
   // double-star states (KH/mhd.cpp:337-377)
-  const R sqrtrol = sq_pos(rolst);
-  const R sqrtror = sq_pos(rorst);
+  R sqrtrol, sqrtror, rsqrtrol, rsqrtror;
+  sq_rsq_pos(rolst, sqrtrol, rsqrtrol);
+  sq_rsq_pos(rorst, sqrtror, rsqrtror);
   const R abbx = fabs(bxs);
   const R signbx = copysign1(bxs);
   sign1 = copysign1(abbx - eps);
@@ -287,8 +311,8 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   const R eerdst = eerst + sqrtror * signbx * (vdbstr - temp) * maxs1;
 
   // flux selection by sign(S_M) (KH/mhd.cpp:381-415)
-  const R slst = (sm - dv(abbx, sqrtrol)) * maxs1;
-  const R srst = (sm + dv(abbx, sqrtror)) * maxs1;
+  const R slst = (sm - abbx * rsqrtrol) * maxs1;
+  const R srst = (sm + abbx * rsqrtror) * maxs1;
   sign1 = copysign1(sm);
   maxs1 = fmax(R(0), sign1);
   mins1 = -fmin(R(0), sign1);
