@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1348434.4 * 1024 * 2.0 + 1310720.6 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1373574.0 * 1024 * 2.0 + 1392896.8 * 1024 * 1.0,
-    (256, 0, 1, "fused_xy"): 732338.4 * 1024 * 2.0 + 1310802.6 * 1024 * 1.0,
-    (256, 0, 1, "fused_z_update"): 1559696.4 * 1024 * 2.0 + 1393829.7 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1353067.9 * 1024 * 2.0 + 1310721.6 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373786.5 * 1024 * 2.0 + 1392897.8 * 1024 * 1.0,
+    (256, 0, 1, "fused_xy"): 732331.7 * 1024 * 2.0 + 1310735.2 * 1024 * 1.0,
+    (256, 0, 1, "fused_z_update"): 1559755.6 * 1024 * 2.0 + 1393848.5 * 1024 * 1.0,
 }
 
 
