@@ -339,8 +339,8 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, cha
   const bool coop_l = coop && (pos > 0 || wrap);
   const bool coop_r = coop && (pos < rw - 1 || wrap);
   const int nb_l = grp + (pos + rw - 1) % rw, nb_r = grp + (pos + 1) % rw;
-  // edge cell `side` (0 first, 1 last) of row rw published by wave wave
-  auto edge_cell = [&](int wave, int rw, int side) { return edgeP + wave * XY_EDGE_R + ((rw & 3) * 2 + side) * NPRIM; };
+  // the edge cell (side 0: first, 1: last) of row jr that wavefront `wave` published
+  auto edge_cell = [&](int wave, int jr, int side) { return edgeP + wave * XY_EDGE_R + ((jr & 3) * 2 + side) * NPRIM; };
   R* const edge_rt = edgeP + wv * XY_EDGE_R + 8 * NPRIM;
 
   // pre-pass: the strip-edge x faces of every row of the chunk without a block neighbour,
