@@ -1,6 +1,7 @@
 """Benchmark: cell-updates/s of the MHD time step on one MI355X, with its HBM roofline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--mode 0|1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256 | --shape NX NY NZ]
+                    [--mode 0|1] [--precision 0|1] [--no-cpu] [--no-serve]
 
 Workload: a 256^3 FP64 Kelvin-Helmholtz shear-layer box (gamma 5/3, Bx = 0.1,
 vx = 0.5 tanh(y/0.05), seeded vy kick; periodic x,z, reflecting y), advanced with the
@@ -16,8 +17,12 @@ the production loop (each step a replayed hipGraph, no events) for comparison.
 roofline: algorithmic bytes of a cell update (read + write the 10 FP64 conserved
 variables = 160 B) per launch of the dominant kernel / that kernel's average launch
 duration in the timed region, against 8 TB/s.
+host_transfer: the reference's host-array crossings timed once (upload of P and U, download
+of P, through the pageable numpy arrays), and the rate if every step made that round trip;
+never part of `value`.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
 host's cores on 16 steps (about 10 s) of the same 256^3 box.
+ensemble / serving: the batched and request-serving measurements (skipped by --no-serve).
 """
 from __future__ import annotations
 
@@ -204,6 +209,20 @@ def main():
     t, _, done = box.run(args.profile_steps, time=t, mode=args.mode)
     graph_ms = 1e3 * (time.perf_counter() - t0) / max(done, 1)
     log(f"[bench] graph replay {graph_ms:.3f} ms/step")
+    # host buffers across the boundary (not part of `value`): the reference's FieldArray
+    # h2d of P and U and the d2h of P, timed once each after a warm-up transfer
+    P_host, U_host = box.download_prim(), box.download_cons()
+    t0 = time.perf_counter()
+    box.upload_prim(P_host)
+    box.upload_cons(U_host)
+    up_ms = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    P_host = box.download_prim()
+    down_ms = 1e3 * (time.perf_counter() - t0)
+    del P_host, U_host
+    pcie = {"upload_P_U_ms": round(up_ms, 2), "download_P_ms": round(down_ms, 2),
+            "cell_updates_per_s_if_each_step_round_trips": cells / ((ms_per_step + up_ms + down_ms) * 1e-3)}
+    log(f"[bench] host transfers: upload P+U {up_ms:.1f} ms, download P {down_ms:.1f} ms")
     dominant = max(prof, key=prof.get)
     dom_ms = prof[dominant]
     step_ms = sum(prof.values())
@@ -231,6 +250,7 @@ def main():
         "roofline": roofline,
         "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
         "graph_replay_ms_per_step": round(graph_ms, 4),
+        "host_transfer": pcie,
         "status": status,
         "sim_time": t,
     }
