@@ -18,7 +18,8 @@ roofline: algorithmic bytes of a cell update (read + write the 10 FP64 conserved
 variables = 160 B) per launch of the dominant kernel / that kernel's average launch
 duration in the timed region, against 8 TB/s.
 host_transfer: the reference's host-array crossings timed once (upload of P and U, download
-of P, through the pageable numpy arrays), and the rate if every step made that round trip;
+of P, between pageable numpy arrays allocated once), and the rate if every step made that
+round trip;
 never part of `value`.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
 host's cores on 16 steps (about 10 s) of the same 256^3 box.
@@ -210,14 +211,15 @@ def main():
     graph_ms = 1e3 * (time.perf_counter() - t0) / max(done, 1)
     log(f"[bench] graph replay {graph_ms:.3f} ms/step")
     # host buffers across the boundary (not part of `value`): the reference's FieldArray
-    # h2d of P and U and the d2h of P, timed once each after a warm-up transfer
+    # h2d of P and U and the d2h of P, timed once each after a warm-up transfer, into host
+    # arrays allocated once and already touched (as the reference's P and U are)
     P_host, U_host = box.download_prim(), box.download_cons()
     t0 = time.perf_counter()
     box.upload_prim(P_host)
     box.upload_cons(U_host)
     up_ms = 1e3 * (time.perf_counter() - t0)
     t0 = time.perf_counter()
-    P_host = box.download_prim()
+    box.download_prim(out=P_host)
     down_ms = 1e3 * (time.perf_counter() - t0)
     del P_host, U_host
     pcie = {"upload_P_U_ms": round(up_ms, 2), "download_P_ms": round(down_ms, 2),

@@ -52,6 +52,43 @@ __device__ __forceinline__ float dv<float>(float a, float b) {
 }
 #endif
 
+// Reciprocals that share one division (MHD_SHARED_RCP=1, default, FP64 only): 1/a = b/(ab)
+// and 1/b = a/(ab), and four at once through 1/(ab) and 1/(cd).  Each result is within a
+// few ulp of the IEEE reciprocal.  Used only where the solver's floors keep every product
+// far inside the FP64 range: densities (taken to lie within 1e-150..1e150), wave-speed
+// differences (|x| >= 1e-20 by construction) and star-region denominators (|x| >= 1e-30).
+// FP32 divides each one, since its range would not hold such products.
+#ifndef MHD_SHARED_RCP
+#define MHD_SHARED_RCP 1
+#endif
+template <typename R>
+__device__ __forceinline__ void recip2(R a, R b, R& ia, R& ib) {
+  if constexpr (MHD_SHARED_RCP && sizeof(R) == 8) {
+    const R r = dv(R(1), a * b);
+    ia = b * r;
+    ib = a * r;
+  } else {
+    ia = dv(R(1), a);
+    ib = dv(R(1), b);
+  }
+}
+template <typename R>
+__device__ __forceinline__ void recip4(R a, R b, R c, R d, R& ia, R& ib, R& ic, R& id) {
+  if constexpr (MHD_SHARED_RCP && sizeof(R) == 8) {
+    R iab, icd;
+    recip2(a * b, c * d, iab, icd);
+    ia = b * iab;
+    ib = a * iab;
+    ic = d * icd;
+    id = c * icd;
+  } else {
+    ia = dv(R(1), a);
+    ib = dv(R(1), b);
+    ic = dv(R(1), c);
+    id = dv(R(1), d);
+  }
+}
+
 // Square root of the hot paths.  MHD_FAST_SQRT=1 (default): FP64 from the hardware
 // reciprocal square root y, one Goldschmidt step (g = x y, h = y/2, r = 1/2 - h g) and one
 // residual correction g + (x - g^2) h: the error of y is squared twice, leaving the result
@@ -123,12 +160,26 @@ __device__ __forceinline__ void sq_rsq_pos(float x, float& s, float& rs) {
 
 // Half the van Leer harmonic limiter of KH/mhd.cpp:89-95: vL(dp, dm) / 2 = dp dm / (dp + dm)
 // where dp dm > 0, else 0.  Scaling by 2 is exact in binary floating point (away from the
-// subnormal range) and commutes with every rounding of the quotient, so q0 +- h equals the
-// reference's q0 +- 0.5 * (2 dp dm / (dp + dm)) bit for bit, without the two scalings.
+// subnormal range) and commutes with every rounding of the quotient, so q0 +- h is the
+// reference's q0 +- 0.5 * (2 dp dm / (dp + dm)) without the two scalings.
+// MHD_LIM_DIV=1 (default): in FP64 the quotient takes the reciprocal and its Newton step but
+// no residual correction, so h is within about two ulp of the IEEE quotient (h enters the
+// states as q0 +- h, a fraction of the cell value).
+#ifndef MHD_LIM_DIV
+#define MHD_LIM_DIV 1
+#endif
 template <typename R>
 __device__ __forceinline__ R half_van_leer(R dp, R dm) {
   const R prod = dp * dm;
-  const R h = dv(prod, dp + dm);
+  R h;
+  if constexpr (MHD_LIM_DIV && MHD_FAST_DIV && sizeof(R) == 8) {
+    const R s = dp + dm;
+    R r = __builtin_amdgcn_rcp(s);
+    r = fma(fma(-s, r, R(1)), r, r);
+    h = prod * r;
+  } else {
+    h = dv(prod, dp + dm);
+  }
   return prod > R(0) ? h : R(0);
 }
 
@@ -144,11 +195,11 @@ struct RState {
 
 // Prim2ConsD for direction D (KH/mhd.cpp:997-1076).  EFLUX=0: energy flux magnetic term
 // |B|^2 B_n as written at KH/mhd.cpp:1049-1050; EFLUX=1: (v.B) B_n.
+// rhoinv: 1/rho, computed by the caller for both states of a face
 template <int D, int EFLUX, typename R>
-__device__ __forceinline__ void prim_to_state(const R* q, RState<R>& s) {
+__device__ __forceinline__ void prim_to_state(const R* q, R rhoinv, RState<R>& s) {
   constexpr int DF = (D + 1) % 3, DB = (D + 2) % 3;
   const R rho = q[nden];
-  const R rhoinv = dv(R(1), rho);
   const R vsq = q[nve1] * q[nve1] + q[nve2] * q[nve2] + q[nve3] * q[nve3];
   const R bsq = q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3];
   const R vn = q[nve1 + D], bn = q[nbm1 + D];
@@ -227,15 +278,20 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   const R sdmr = fmax(tiny, sr - sm);
   const R ptst = (rosdr * ptl - rosdl * ptr + rosdl * rosdr * (vxr - vxl)) * temp;
 
-  // left star state (KH/mhd.cpp:268-302)
+  // star-state denominators of both sides and their reciprocals (KH/mhd.cpp:268-276, 304-312)
   const R bxs = R(0.5) * (L.bu + Rt.bu);
   const R bxsq = bxs * bxs;
-  R temp_fst = rosdl * sdml - bxsq;
-  R sign1 = copysign1(fabs(temp_fst) - eps);
-  R maxs1 = fmax(R(0), sign1);
-  R mins1 = fmin(R(0), sign1);
-  R itf = dv(R(1), temp_fst + mins1);
-  const R isdml = dv(R(1), sdml);
+  const R tfl = rosdl * sdml - bxsq;
+  const R sgl = copysign1(fabs(tfl) - eps);
+  const R tfr = rosdr * sdmr - bxsq;
+  const R sgr = copysign1(fabs(tfr) - eps);
+  R itfl, isdml, itfr, isdmr;
+  recip4(tfl + fmin(R(0), sgl), sdml, tfr + fmin(R(0), sgr), sdmr, itfl, isdml, itfr, isdmr);
+
+  // left star state (KH/mhd.cpp:268-302)
+  R maxs1 = fmax(R(0), sgl);
+  R mins1 = fmin(R(0), sgl);
+  R itf = itfl;
   temp = bxs * (sdl - sdml) * itf;
   const R rolst = maxs1 * (rosdl * isdml) - mins1 * rol;
   const R sclst = rolst * L.X;  // rolst/rol*scl with scl/rol = X
@@ -254,12 +310,9 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
       mins1 * eel;
 
   // right star state (KH/mhd.cpp:304-333)
-  temp_fst = rosdr * sdmr - bxsq;
-  sign1 = copysign1(fabs(temp_fst) - eps);
-  maxs1 = fmax(R(0), sign1);
-  mins1 = fmin(R(0), sign1);
-  itf = dv(R(1), temp_fst + mins1);
-  const R isdmr = dv(R(1), sdmr);
+  maxs1 = fmax(R(0), sgr);
+  mins1 = fmin(R(0), sgr);
+  itf = itfr;
   temp = bxs * (sdr - sdmr) * itf;
   const R rorst = maxs1 * (rosdr * isdmr) - mins1 * ror;
   const R scrst = rorst * Rt.X;
@@ -283,7 +336,7 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   sq_rsq_pos(rorst, sqrtror, rsqrtror);
   const R abbx = fabs(bxs);
   const R signbx = copysign1(bxs);
-  sign1 = copysign1(abbx - eps);
+  R sign1 = copysign1(abbx - eps);
   maxs1 = fmax(R(0), sign1);
   mins1 = -fmin(R(0), sign1);
   const R invsumro = dv(maxs1, sqrtrol + sqrtror);
@@ -344,8 +397,10 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
 template <int D, int EFLUX, typename R>
 __device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double chg, double hchg, double* F) {
   RState<R> sl, sr;
-  prim_to_state<D, EFLUX, R>(ql, sl);
-  prim_to_state<D, EFLUX, R>(qr, sr);
+  R rinvl, rinvr;
+  recip2(ql[nden], qr[nden], rinvl, rinvr);
+  prim_to_state<D, EFLUX, R>(ql, rinvl, sl);
+  prim_to_state<D, EFLUX, R>(qr, rinvr, sr);
   RFlux<R> nf;
   hlld<R>(sl, sr, nf);
   F[mden] = double(nf.d);

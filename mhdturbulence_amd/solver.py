@@ -115,10 +115,19 @@ class Box:
         assert U.shape == self._shape(NCONS), U.shape
         capi.check(self.L.mhd_upload_cons(self.h, _dp(U)))
 
-    def download_prim(self) -> np.ndarray:
-        P = np.zeros(self._shape(NPRIM))
+    def download_prim(self, out: np.ndarray = None) -> np.ndarray:
+        """P in the host layout; out (C-contiguous float64 of that shape) is filled and returned
+        when given, so a loop that downloads often reuses one array, as the reference's P does"""
+        P = self._out(out, NPRIM)
         capi.check(self.L.mhd_download_prim(self.h, _dp(P)))
         return P
+
+    def _out(self, out, nv):
+        if out is None:
+            return np.zeros(self._shape(nv))
+        if out.shape != self._shape(nv) or out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be C-contiguous float64 of shape {self._shape(nv)}")
+        return out
 
     def snapshot_begin(self):
         """queue a copy of the current primitive state to host memory; later steps overlap it"""
@@ -133,8 +142,8 @@ class Box:
         view = np.ctypeslib.as_array(ptr, shape=(int(np.prod(shape)),)).reshape(shape)
         return view.copy() if copy else view
 
-    def download_cons(self) -> np.ndarray:
-        U = np.zeros(self._shape(NCONS))
+    def download_cons(self, out: np.ndarray = None) -> np.ndarray:
+        U = self._out(out, NCONS)
         capi.check(self.L.mhd_download_cons(self.h, _dp(U)))
         return U
 

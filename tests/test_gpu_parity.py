@@ -67,6 +67,13 @@ def test_upload_download_roundtrip(orc):
     o, b = pair(orc, n=(12, 10, 8))
     np.testing.assert_array_equal(b.download_prim(), o.P)
     np.testing.assert_array_equal(b.download_cons(), o.U)
+    # into caller arrays that hold other values: every element is overwritten
+    P, U = np.full_like(o.P, np.nan), np.full_like(o.U, -1.0)
+    assert b.download_prim(out=P) is P and b.download_cons(out=U) is U
+    np.testing.assert_array_equal(P, o.P)
+    np.testing.assert_array_equal(U, o.U)
+    with pytest.raises(ValueError):
+        b.download_prim(out=U)
 
 
 @pytest.mark.parametrize("problem", ["kh", "shear"])
