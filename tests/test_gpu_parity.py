@@ -128,6 +128,84 @@ def test_reference_functions_one_by_one(orc, eflux, problem):
     assert np.all(rel_l1(o.interior(b.download_prim()), o.interior(o.P)) < 1e-14)
 
 
+def _wide_state(o, smooth: bool, gam: float):
+    """primitive states over many decades, written into the oracle's P (ghosts included).
+    smooth=False: each cell drawn independently (density 1e-4..1e3, pressure 1e-3..1e4,
+    |v| up to 5, |B| up to 20), for face-by-face flux checks.  smooth=True: density and
+    pressure waves of +-3.5 decades across the box with 25 % cell noise, Mach and plasma beta
+    of order one, so that a few steps stay physical."""
+    rng = np.random.default_rng(20261203)
+    P = o.P
+    K, J, _, I = P.shape
+    k, j, i = np.meshgrid(np.arange(K), np.arange(J), np.arange(I), indexing="ij")
+    if smooth:
+        rho = 10.0 ** (3.5 * np.sin(2 * np.pi * i / 20) * np.cos(2 * np.pi * k / 16) + 0.1 * rng.uniform(-1, 1, i.shape))
+        p = 10.0 ** (3.5 * np.cos(2 * np.pi * j / 18 + 0.3) * np.sin(2 * np.pi * i / 20 + 1.0) +
+                     0.1 * rng.uniform(-1, 1, i.shape))
+        vs, bs = 0.5 * np.sqrt(gam * p / rho), np.sqrt(p)
+    else:
+        rho = 10.0 ** rng.uniform(-4, 3, i.shape)
+        p = 10.0 ** rng.uniform(-3, 4, i.shape)
+        vs, bs = 5.0, 20.0
+    P[:, :, 0] = rho
+    for c in (1, 2, 3):
+        P[:, :, c] = rng.uniform(-1, 1, i.shape) * vs
+    P[:, :, 4] = p / ((gam - 1.0) * rho)
+    P[:, :, 5] = p
+    P[:, :, 6] = np.sqrt(gam * p / rho)
+    for c in (7, 8, 9):
+        P[:, :, c] = rng.uniform(-1, 1, i.shape) * bs
+    P[:, :, 10] = rng.uniform(-1, 1, i.shape)
+    P[:, :, 11] = rng.uniform(0, 1, i.shape)
+    o.prim_to_cons_all()
+
+
+def test_wide_range_states(orc):
+    """states over many decades: the fluxes of every direction face by face, then three
+    fused and three staged steps, against the oracle.  Reaches the HLLD's star-state floors
+    and the products its shared reciprocals form across the range the solver meets."""
+    say("wide-range states 20x18x16")
+    n = (20, 18, 16)
+    kw = dict(energy_flux=1, gam=5.0 / 3.0, bc=KH_BC)
+    o, b = pair(orc, n=n, problem="kh", upload=False, **kw)
+    _wide_state(o, False, kw["gam"])
+    b.upload_prim(o.P)
+    b.upload_cons(o.U)
+    dt_o, dt_g = o.control_timestep(), b.control_timestep()
+    assert dt_g == pytest.approx(dt_o, rel=1e-15)
+    o.set_boundary()
+    b.set_boundary_condition()
+    ch_o, ch_g = o.evaluate_ch(), b.evaluate_ch()
+    assert ch_g == pytest.approx(ch_o, rel=1e-15)
+    for d in (1, 2, 3):
+        o.numerical_flux(d, ch_o)
+        b.get_numerical_flux(d, ch_o)
+        Fo, Fg = o.F(d), b.download_flux(d)
+        sl = [slice(2, -3)] * 3
+        sl[2 - (d - 1)] = slice(2, -2)
+        s = (sl[0], sl[1], slice(None), sl[2])
+        # each face against its own largest component: the faces span many decades
+        scale = np.abs(Fo[s]).max(axis=2, keepdims=True) + 1e-300
+        err = (np.abs(Fg[s] - Fo[s]) / scale).max()
+        assert np.isfinite(Fg[s]).all()
+        assert err < 1e-11, (d, err)
+    b.close()
+    for mode in (0, 1):
+        o2, b2 = pair(orc, n=n, problem="kh", upload=False, **kw)
+        _wide_state(o2, True, kw["gam"])
+        b2.upload_prim(o2.P)
+        b2.upload_cons(o2.U)
+        t_o, log_o = o2.run(3)
+        t_g, log_g, done = b2.run(3, mode=mode)
+        assert done == 3
+        np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+        errP = rel_l1(o2.interior(b2.download_prim()), o2.interior(o2.P))
+        errU = rel_l1(o2.interior(b2.download_cons()), o2.interior(o2.U))
+        say(f"  mode {mode}: max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+        assert np.all(errP < 1e-12) and np.all(errU < 1e-12), (mode, errP, errU)
+        b2.close()
+
+
 def _run_parity(orc, n, steps, mode, problem="kh", **kw):
     o, b = pair(orc, n=n, problem=problem, **kw)
     t_o, log_o = o.run(steps)
