@@ -51,6 +51,7 @@ struct orc_ctx {
   int itot, jtot, ktot, is, ie, js, je, ks, ke;
   std::vector<double> x1a, x1b, x2a, x2b, x3a, x3b;
   std::vector<double> P, U, F[3];
+  std::vector<double> U0;  // the state at the start of a two-stage step
   size_t idx(int nv, int n, int k, int j, int i) const {
     return ((static_cast<size_t>(k) * jtot + j) * nv + n) * itot + i;
   }
@@ -674,19 +675,43 @@ void orc_update_primitive(orc_ctx* c) {
 }
 
 // The loop body of KH/main.cpp:229-259, nsteps times.
+void orc_save_cons(orc_ctx* c) { c->U0 = c->U; }
+
+void orc_average_saved(orc_ctx* c) {
+#pragma omp parallel for collapse(2)
+  for (int k = c->ks; k <= c->ke; ++k)
+    for (int j = c->js; j <= c->je; ++j)
+      for (int i = c->is; i <= c->ie; ++i)
+        for (int m = 0; m < NCONS; ++m) {
+          const size_t x = c->idx(NCONS, m, k, j, i);
+          c->U[x] = 0.5 * (c->U0[x] + c->U[x]);
+        }
+}
+
+// The loop body of KH/main.cpp:229-259.  stages = 2 wraps the reference's forward-Euler
+// stage E(U) = DampPsi(U - dt div F(U)) (the calls below, KH/main.cpp:246-257) in the
+// two-stage strong-stability-preserving Runge-Kutta step of Shu & Osher (J. Comput. Phys.
+// 77:439, 1988): U* = E(U^n), U^{n+1} = (U^n + E(U*)) / 2, both stages with the dt and c_h
+// of U^n.  The reference itself has no Runge-Kutta step.
 double orc_run(orc_ctx* c, int nsteps, double t0, double* dt_log) {
   double t = t0;
+  const bool rk2 = c->cfg.stages == 2;
   for (int s = 0; s < nsteps; ++s) {
     const double dt = orc_control_timestep(c);
     if (dt_log) dt_log[s] = dt;
     orc_set_boundary(c);
     const double chg = orc_evaluate_ch(c);
-    orc_numerical_flux(c, 1, chg);
-    orc_numerical_flux(c, 2, chg);
-    orc_numerical_flux(c, 3, chg);
-    orc_update_conserv(c, dt);
-    orc_damp_psi(c, dt, chg);
-    orc_update_primitive(c);
+    if (rk2) orc_save_cons(c);
+    for (int stage = 0; stage < (rk2 ? 2 : 1); ++stage) {
+      if (stage == 1) orc_set_boundary(c);
+      orc_numerical_flux(c, 1, chg);
+      orc_numerical_flux(c, 2, chg);
+      orc_numerical_flux(c, 3, chg);
+      orc_update_conserv(c, dt);
+      orc_damp_psi(c, dt, chg);
+      if (stage == 1) orc_average_saved(c);
+      orc_update_primitive(c);
+    }
     t += dt;
   }
   return t;

@@ -34,6 +34,7 @@ typedef struct orc_config {
   double alphabp;          /* psi damping constant (reference: 0.1) */
   int energy_flux;         /* 0: (E+pt)vn - |B|^2 Bn (src_cpp_kokkos_host); 1: (E+pt)vn - (v.B) Bn */
   int grid_shift;          /* 1: x1a(i)=dx*(i-(ngh+1))+xmin (C++ variants); 0: dx*(i-ngh)+xmin */
+  int stages;              /* 0 or 1: forward Euler, the reference's step; 2: two-stage SSP Runge-Kutta */
 } orc_config;
 
 typedef struct orc_ic_params {
@@ -69,6 +70,10 @@ void   orc_numerical_flux(orc_ctx* c, int dir, double chg);
 void   orc_update_conserv(orc_ctx* c, double dt);
 void   orc_damp_psi(orc_ctx* c, double dt, double chg);
 void   orc_update_primitive(orc_ctx* c);
+/* U <- (U0 + U) / 2 on interior cells, U0 the copy orc_save_cons took (the second stage of
+ * the two-stage Runge-Kutta step) */
+void   orc_save_cons(orc_ctx* c);
+void   orc_average_saved(orc_ctx* c);
 /* nsteps of the KH loop body; returns the simulation time; dt_log (may be NULL) gets each dt */
 double orc_run(orc_ctx* c, int nsteps, double t0, double* dt_log);
 /* out[0]=volume, out[1]=mix, out[2]=sqrt(<vy^2>), out[3]=max|divB| */

@@ -25,7 +25,7 @@ class OrcConfig(C.Structure):
         ("bc", C.c_int * 6),
         ("xmin", C.c_double * 3), ("xmax", C.c_double * 3),
         ("gam", C.c_double), ("cfl", C.c_double), ("alphabp", C.c_double),
-        ("energy_flux", C.c_int), ("grid_shift", C.c_int),
+        ("energy_flux", C.c_int), ("grid_shift", C.c_int), ("stages", C.c_int),
     ]
 
 
@@ -64,8 +64,8 @@ def lib():
         L.orc_grid.restype = dp
         L.orc_grid.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_generate_problem_kh.argtypes = [C.c_void_p, C.c_double, C.c_double]
-        for name in ("orc_generate_grid", "orc_prim_to_cons_all",
-                     "orc_set_boundary", "orc_update_primitive"):
+        for name in ("orc_generate_grid", "orc_prim_to_cons_all", "orc_set_boundary", "orc_update_primitive",
+                     "orc_save_cons", "orc_average_saved"):
             getattr(L, name).argtypes = [C.c_void_p]
         L.orc_generate_problem_shear.argtypes = [C.c_void_p, C.POINTER(OrcIcParams)]
         L.orc_control_timestep.restype = C.c_double
@@ -94,7 +94,7 @@ def _dp(a: np.ndarray):
 
 
 def make_config(nx, ny, nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -1.0, -0.5), xmax=(0.5, 1.0, 0.5),
-                gam=1.4, cfl=0.05, alphabp=0.1, energy_flux=0, grid_shift=1) -> OrcConfig:
+                gam=1.4, cfl=0.05, alphabp=0.1, energy_flux=0, grid_shift=1, stages=1) -> OrcConfig:
     cfg = OrcConfig()
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for i, v in enumerate(bc):
@@ -102,7 +102,7 @@ def make_config(nx, ny, nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -1.0, -0.5), xmax
     for i in range(3):
         cfg.xmin[i], cfg.xmax[i] = xmin[i], xmax[i]
     cfg.gam, cfg.cfl, cfg.alphabp = gam, cfl, alphabp
-    cfg.energy_flux, cfg.grid_shift = energy_flux, grid_shift
+    cfg.energy_flux, cfg.grid_shift, cfg.stages = energy_flux, grid_shift, stages
     return cfg
 
 
@@ -178,6 +178,13 @@ class Oracle:
 
     def update_primitive(self):                  # UpdatePrimitvP
         self.L.orc_update_primitive(self.h)
+
+    # the two-stage Runge-Kutta step's own parts (no reference counterpart)
+    def save_cons(self):
+        self.L.orc_save_cons(self.h)
+
+    def average_saved(self):
+        self.L.orc_average_saved(self.h)
 
     def run(self, nsteps: int, t0: float = 0.0):
         log = np.zeros(max(nsteps, 1))

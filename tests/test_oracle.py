@@ -350,13 +350,31 @@ def test_mirror_symmetry_y(orc):
     np.testing.assert_allclose(Pi, flip, rtol=0, atol=1e-13)
 
 
-def _linear_wave_error(orc, n, wave):
+def _rk2_step(o, dt, ch):
+    """one two-stage SSP Runge-Kutta step from the oracle's reference functions:
+    U* = E(U^n), U^{n+1} = (U^n + E(U*)) / 2, E = the reference's forward-Euler stage"""
+    o.save_cons()
+    for stage in (0, 1):
+        if stage:
+            o.set_boundary()
+        for d in (1, 2, 3):
+            o.numerical_flux(d, ch)
+        o.update_conserv(dt)
+        o.damp_psi(dt, ch)
+        if stage:
+            o.average_saved()
+        o.update_primitive()
+
+
+def _linear_wave_error(orc, n, wave, rk2=False):
     """L1 error of a small-amplitude wave along x after a quarter crossing.
 
     The reference advances with one forward-Euler stage per step (first order in time), so
-    the CFL number is scaled with 1/n to expose the second-order spatial discretisation."""
+    the CFL number is scaled with 1/n to expose the second-order spatial discretisation.
+    rk2: the two-stage Runge-Kutta step at a fixed CFL number of 0.3, second order in time."""
     o = orc.Oracle(nx=n, ny=4, nz=4, bc=(1, 1, 1, 1, 1, 1), xmin=(0, 0, 0), xmax=(1, 4.0 / n, 4.0 / n),
-                   gam=5.0 / 3.0, cfl=0.1 * 32.0 / n, energy_flux=1, grid_shift=0)
+                   gam=5.0 / 3.0, cfl=0.3 if rk2 else 0.1 * 32.0 / n, energy_flux=1, grid_shift=0,
+                   stages=2 if rk2 else 1)
     o.generate_grid()
     gam = 5.0 / 3.0
     # p0 = 0.6/gam keeps c_s != c_A: at c_s = c_A the fast-speed discriminant of the reference
@@ -399,11 +417,14 @@ def _linear_wave_error(orc, n, wave):
             dt = t_end - t
         o.set_boundary()
         ch = o.evaluate_ch()
-        for d in (1, 2, 3):
-            o.numerical_flux(d, ch)
-        o.update_conserv(dt)
-        o.damp_psi(dt, ch)
-        o.update_primitive()
+        if rk2:
+            _rk2_step(o, dt, ch)
+        else:
+            for d in (1, 2, 3):
+                o.numerical_flux(d, ch)
+            o.update_conserv(dt)
+            o.damp_psi(dt, ch)
+            o.update_primitive()
         t += dt
     xs = x[2:2 + n] - speed * t_end
     if wave == "sound":
@@ -420,6 +441,57 @@ def test_second_order_convergence(orc, wave):
     e = [_linear_wave_error(orc, n, wave) for n in (32, 64, 128)]
     orders = np.log2(np.array(e[:-1]) / np.array(e[1:]))
     assert np.all(orders > 1.6), (e, orders)
+
+
+@pytest.mark.parametrize("wave", ["sound", "alfven"])
+def test_rk2_second_order_at_fixed_cfl(orc, wave):
+    """the two-stage step converges at second order with the time step tied to the cell
+    size (CFL 0.3); a dropped stage or average would leave first order in time"""
+    e = [_linear_wave_error(orc, n, wave, rk2=True) for n in (32, 64, 128)]
+    orders = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    assert np.all(orders > 1.7), (e, orders)
+
+
+def test_rk2_psi_damping_closed_form(orc):
+    """a uniform state with uniform psi: every flux difference vanishes and only DampPsi acts,
+    so each two-stage step multiplies psi by 1 - x + x^2/2 (x = dt 0.1 c_h / min(dx,dy,dz)),
+    the second-order Taylor polynomial of exp(-x); one Euler step multiplies by 1 - x"""
+    for stages in (1, 2):
+        o = _box(orc, n=(8, 6, 4), xmin=(0, 0, 0), xmax=(1, 0.75, 1.0), gam=5.0 / 3.0, stages=stages, cfl=0.3,
+                 bc=(1, 1, 1, 1, 1, 1))
+        P = o.P
+        rho, p, gam = 1.2, 0.9, 5.0 / 3.0
+        P[:, :, 0] = rho
+        P[:, :, 1:4] = np.array([0.3, -0.2, 0.1])[None, None, :, None]
+        P[:, :, 5] = p
+        P[:, :, 4] = p / ((gam - 1) * rho)
+        P[:, :, 6] = np.sqrt(gam * p / rho)
+        P[:, :, 7:10] = np.array([0.2, 0.1, -0.3])[None, None, :, None]
+        P[:, :, 10] = 0.05
+        P[:, :, 11] = 0.5
+        o.prim_to_cons_all()
+        dt, ch = o.control_timestep(), o.evaluate_ch()
+        x = dt * 0.1 * ch / min(1 / 8, 0.75 / 6, 1 / 4)
+        g = 1 - x + x * x / 2 if stages == 2 else 1 - x
+        t, log = o.run(5)
+        np.testing.assert_allclose(log, dt, rtol=1e-14)
+        np.testing.assert_allclose(o.interior(o.P)[10], 0.05 * g ** 5, rtol=1e-13)
+        np.testing.assert_allclose(o.interior(o.P)[0], rho, rtol=1e-14)
+
+
+def test_rk2_run_is_the_two_stage_composition(orc):
+    """orc_run with stages = 2 performs exactly the step the convergence test verifies"""
+    kw = dict(n=(12, 10, 8), gam=5.0 / 3.0, cfl=0.3, energy_flux=1, stages=2)
+    a, b = _box(orc, **kw), _box(orc, **kw)
+    for o in (a, b):
+        o.generate_problem_shear(bx0=0.1, a=0.08, amp=0.05)
+    a.run(4)
+    for _ in range(4):
+        dt = b.control_timestep()
+        b.set_boundary()
+        _rk2_step(b, dt, b.evaluate_ch())
+    np.testing.assert_array_equal(a.U, b.U)
+    np.testing.assert_array_equal(a.interior(a.P), b.interior(b.P))
 
 
 def test_prim_cons_roundtrip(orc):
