@@ -48,6 +48,11 @@ typedef struct mhd_config {
                               0: dx*(i-ngh)+xmin */
   int precision;           /* 0: FP64 throughout; 1: reconstruction + HLLD in FP32 */
   int device;              /* HIP device ordinal */
+  int stages;              /* 0 or 1: one forward-Euler stage per step, the reference's
+                              (src_cpp_kokkos_host/main.cpp:229-259); 2: the two-stage SSP
+                              Runge-Kutta step over that stage, U* = E(U^n),
+                              U^{n+1} = (U^n + E(U*))/2, dt and c_h from U^n (no counterpart
+                              in the reference) */
 } mhd_config;
 
 /* single-shear-layer initial state (see mhd_generate_problem_shear) */

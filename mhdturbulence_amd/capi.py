@@ -34,6 +34,7 @@ class MhdConfig(C.Structure):
         ("xmin", C.c_double * 3), ("xmax", C.c_double * 3),
         ("gam", C.c_double), ("cfl", C.c_double), ("alphabp", C.c_double),
         ("energy_flux", C.c_int), ("grid_shift", C.c_int), ("precision", C.c_int), ("device", C.c_int),
+        ("stages", C.c_int),
     ]
 
 

@@ -237,11 +237,22 @@ int rebuild(mhd_batch* b) {
   drop_graph(b);
   if (b->nlive > 0) {
     const int ef = b->base.energy_flux, pr = b->base.precision;
-    auto xy = [&](int g, hipStream_t st) {
-      mhd::launch_batch_xy(b->d_args, b->d_tab[0] + b->goff[g][0], b->gblk[g][0], b->glds[g], ef, pr, st);
+    const bool rk2 = b->base.stages == 2;
+    auto xy_at = [&](int g, hipStream_t st, int stage) {
+      mhd::launch_batch_xy(b->d_args, b->d_tab[0] + b->goff[g][0], b->gblk[g][0], b->glds[g], ef, pr, st, stage);
     };
+    auto z_at = [&](int g, hipStream_t st, int stage) {
+      mhd::launch_batch_z(b->d_args, b->d_tab[1] + b->goff[g][1], b->gblk[g][1], ef, pr, st, stage);
+    };
+    // first launch of a group's iteration, then the rest of it: one Euler stage, or the two
+    // Runge-Kutta stages (U0 -> U1, then U1 -> U0 averaged with U0)
+    auto xy = [&](int g, hipStream_t st) { xy_at(g, st, rk2 ? 1 : 0); };
     auto z_end = [&](int g, hipStream_t st) {
-      mhd::launch_batch_z(b->d_args, b->d_tab[1] + b->goff[g][1], b->gblk[g][1], ef, pr, st);
+      z_at(g, st, rk2 ? 1 : 0);
+      if (rk2) {
+        xy_at(g, st, 2);
+        z_at(g, st, 2);
+      }
       mhd::launch_batch_step_end(b->d_args, b->d_live + b->lofs[g], b->glive[g], b->d_dt_last, st);
     };
     BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -328,6 +339,7 @@ int mhd_batch_create(const mhd_config* base, int max_boxes, uint64_t arena_bytes
   for (int q = 0; q < 6; ++q)
     if (base->bc[q] < 1 || base->bc[q] > 3) return fail(MHD_ERR_ARG, "boundary codes are 1, 2 or 3");
   if (!(base->gam > 1.0) || !(base->cfl > 0.0)) return fail(MHD_ERR_ARG, "gam > 1 and cfl > 0 required");
+  if (base->stages < 0 || base->stages > 2) return fail(MHD_ERR_ARG, "stages is 0 or 1 (forward Euler) or 2");
   int ndev = 0;
   BHIP(hipGetDeviceCount(&ndev));
   if (base->device < 0 || base->device >= ndev) return fail(MHD_ERR_ARG, "no such HIP device");
@@ -553,7 +565,9 @@ int mhd_batch_finish_box(mhd_batch* b, int id, double* diag4, double* P_host) {
   StepScalars h;
   BHIP(hipMemcpyAsync(&h, b->d_sc + id, sizeof h, hipMemcpyDeviceToHost, b->stream));
   BHIP(hipStreamSynchronize(b->stream));
-  if (h.iter & 1) {
+  // one-stage iterations leave the state in U1 after an odd number of them (two-stage ones
+  // end in U0)
+  if (b->base.stages != 2 && (h.iter & 1)) {
     BHIP(hipMemcpyAsync(x.U0, x.U1, mhd::NCONS * x.g.vstride * 8, hipMemcpyDeviceToDevice, b->stream));
     // the state is in U0 again; a later step must read U0 first
     int z = 0;

@@ -57,7 +57,7 @@ struct mhd_ctx {
   double* P = nullptr;
   double* U = nullptr;
   double* F[3] = {nullptr, nullptr, nullptr};
-  double* U2 = nullptr;  // second conserved buffer of the fused loop (ping-pong)
+  double* U2 = nullptr;  // second conserved buffer: fused loop ping-pong, or U^n of a staged two-stage step
   double* grid[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // x1a x1b x2a x2b x3a x3b
   double* staging = nullptr;
   double* diag = nullptr;
@@ -160,8 +160,11 @@ int check_cfg(const mhd_config* cfg) {
   if (!(cfg->cfl > 0.0)) return fail(MHD_ERR_ARG, "cfl must be positive");
   if (cfg->energy_flux < 0 || cfg->energy_flux > 1) return fail(MHD_ERR_ARG, "energy_flux is 0 or 1");
   if (cfg->precision < 0 || cfg->precision > 1) return fail(MHD_ERR_ARG, "precision is 0 or 1");
+  if (cfg->stages < 0 || cfg->stages > 2) return fail(MHD_ERR_ARG, "stages is 0 or 1 (forward Euler) or 2");
   return MHD_OK;
 }
+
+bool two_stage(const mhd_ctx* c) { return c->cfg.stages == 2; }
 
 int read_scalars(mhd_ctx* c, StepScalars* h) {
   HIPCHK(hipMemcpyAsync(h, c->sc, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
@@ -456,18 +459,26 @@ int mhd_update_primitive(mhd_ctx* c) {
 // Launches of one loop iteration; every argument is iteration-invariant (the kernels read
 // the iteration's slots from c->sc), so one iteration is captured once and replayed.
 // marks (may be null) receives an event after each launch, for per-kernel timing.
+// stages = 2: the two-stage Runge-Kutta step, U* = E(U^n), U^{n+1} = (U^n + E(U*)) / 2 with
+// E the reference's forward-Euler stage and the dt and c_h of U^n in both stages.
 static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
   const Geom& g = c->g;
+  const bool rk2 = two_stage(c);
   int q = 0;
   auto mark = [&]() {
     if (marks) (void)hipEventRecord(marks[q], c->stream);
     ++q;
   };
   if (mode == 0) {
-    mhd::launch_fused_xy(c->fa, c->stream);
-    mark();
-    mhd::launch_fused_z(c->fa, c->stream);
-    mark();
+    mhd::FusedArgs a = c->fa;
+    a.stage = rk2 ? 1 : 0;
+    for (int st = 0; st < (rk2 ? 2 : 1); ++st) {
+      if (st == 1) a.stage = 2;
+      mhd::launch_fused_xy(a, c->stream);
+      mark();
+      mhd::launch_fused_z(a, c->stream);
+      mark();
+    }
     mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 1, c->stream);
     mark();
     HIPCHK(hipGetLastError());
@@ -475,17 +486,20 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
   }
   mhd::launch_reduce_prim(g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
   mark();
-  mhd::launch_boundary(g, c->cfg.bc, c->P, mhd::NPRIM, c->stream);
-  mark();
-  for (int d = 1; d <= 3; ++d) {
-    mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
+  for (int st = 0; st < (rk2 ? 2 : 1); ++st) {
+    mhd::launch_boundary(g, c->cfg.bc, c->P, mhd::NPRIM, c->stream);
+    mark();
+    for (int d = 1; d <= 3; ++d) {
+      mhd::launch_flux(g, d, c->cfg.energy_flux, c->cfg.precision, c->P, c->F[d - 1], 0.0, c->sc, c->stream);
+      mark();
+    }
+    mhd::launch_update(g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, 0.0, c->cfg.cfl,
+                       c->cfg.alphabp, c->sc, c->stream, rk2 && st == 0 ? c->U2 : nullptr,
+                       rk2 && st == 1 ? c->U2 : nullptr);
+    mark();
+    mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
     mark();
   }
-  mhd::launch_update(g, c->grid[0], c->grid[2], c->grid[4], c->F[0], c->F[1], c->F[2], c->U, 0.0, c->cfg.cfl,
-                     c->cfg.alphabp, c->sc, c->stream);
-  mark();
-  mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
-  mark();
   mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 0, c->stream);
   mark();
   HIPCHK(hipGetLastError());
@@ -497,7 +511,20 @@ static const char* kernel_names(int mode) {
   return "reduce_dt_ch,boundary,flux_x,flux_y,flux_z,update_damp,update_prim,step_end";
 }
 
-static int launches_per_step(int mode) { return mode == 0 ? 3 : 8; }
+static int kernel_kinds(int mode) { return mode == 0 ? 3 : 8; }
+
+static int launches_per_step(const mhd_ctx* c, int mode) {
+  const int n = kernel_kinds(mode);
+  return two_stage(c) ? (mode == 0 ? n + 2 : n + 6) : n;
+}
+
+// the entry of kernel_names(mode) that launch q of an iteration times (both Runge-Kutta
+// stages of a kernel add up in one entry)
+static int launch_kind(const mhd_ctx* c, int mode, int q) {
+  if (!two_stage(c)) return q;
+  if (mode == 0) return q < 4 ? q % 2 : 2;
+  return q == 0 ? 0 : q == 13 ? 7 : 1 + (q - 1) % 6;
+}
 
 // loop state and buffers for nsteps iterations starting at *time
 static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
@@ -513,6 +540,7 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
   }
   if (mode == 0) CHKRET(ensure_fused(c));
   else CHKRET(ensure_flux(c));
+  if (mode == 1 && two_stage(c)) CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
   double tt[2] = {time, tmax};
   HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
   int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
@@ -533,8 +561,9 @@ static int finish_run(mhd_ctx* c, int mode, int nsteps, double* time, double* dt
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   if (mode == 0) {
-    // the state sits in U2 after an odd number of iterations; P is recovered from it
-    if (h.iter & 1)
+    // one-stage iterations leave the state in U2 after an odd number of them; P is
+    // recovered from it (two-stage iterations end in U)
+    if (!two_stage(c) && (h.iter & 1))
       HIPCHK(hipMemcpyAsync(c->U, c->U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
                             c->stream));
     mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
@@ -576,14 +605,14 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
-  c->last_launches = launches_per_step(mode) * nsteps;
+  c->last_launches = launches_per_step(c, mode) * nsteps;
   return MHD_OK;
 }
 
 int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int cap, int* nkernels) {
   if (!c || !avg_ms || !nkernels || nsteps < 1) return fail(MHD_ERR_ARG, "bad profile arguments");
-  const int nk = launches_per_step(mode);
-  if (cap < nk) return fail(MHD_ERR_ARG, "avg_ms too small");
+  const int nk = launches_per_step(c, mode), nkind = kernel_kinds(mode);
+  if (cap < nkind) return fail(MHD_ERR_ARG, "avg_ms too small");
   double t = 0.0;
   CHKRET(prepare_run(c, nsteps, 0.0, mode, t));
   // one event before the first launch and one after every launch of every step, all
@@ -595,20 +624,20 @@ int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int ca
   if (rc == MHD_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventRecord");
   for (int s = 0; s < nsteps && rc == MHD_OK; ++s) rc = issue_step(c, mode, ev.data() + 1 + (size_t)s * nk);
   if (rc == MHD_OK && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventSynchronize");
-  std::vector<double> acc(nk, 0.0);
+  std::vector<double> acc(nkind, 0.0);
   if (rc == MHD_OK)
     for (int s = 0; s < nsteps; ++s)
       for (int q = 0; q < nk; ++q) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, ev[(size_t)s * nk + q], ev[(size_t)s * nk + q + 1]);
-        acc[q] += ms;
+        acc[launch_kind(c, mode, q)] += ms;
       }
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (rc != MHD_OK) return rc;
   CHKRET(finish_run(c, mode, nsteps, &t, nullptr, nullptr));
-  for (int q = 0; q < nk; ++q) avg_ms[q] = acc[q] / nsteps;
-  *nkernels = nk;
+  for (int q = 0; q < nkind; ++q) avg_ms[q] = acc[q] / nsteps;
+  *nkernels = nkind;
   return MHD_OK;
 }
 

@@ -77,9 +77,11 @@ void launch_flux(const Geom& g, int dir, int eflux, int mixed, const double* P, 
                  const StepScalars* sc, hipStream_t s);
 // U -= dt*(dFx/dx + dFy/dy + dFz/dz).  sc != nullptr: dt = cfl*dtmin and c_h from the ring, the
 // psi damping is applied in the same pass, and the update is skipped once the loop stopped.
+// save != nullptr: the state before the update is copied there (the first Runge-Kutta stage);
+// avg != nullptr: the result is averaged with avg, (avg + U) / 2 (the second stage).
 void launch_update(const Geom& g, const double* x1a, const double* x2a, const double* x3a, const double* Fx,
                    const double* Fy, const double* Fz, double* U, double dt, double cfl, double alphabp,
-                   const StepScalars* sc, hipStream_t s);
+                   const StepScalars* sc, hipStream_t s, double* save = nullptr, const double* avg = nullptr);
 void launch_damp(const Geom& g, const double* x1a, const double* x2a, const double* x3a, double* U, double dt,
                  double chg, double alphabp, hipStream_t s);
 void launch_update_prim(const Geom& g, double gam, const double* U, double* P, const StepScalars* sc,

@@ -287,14 +287,14 @@ __host__ __device__ constexpr int xy_edge_len(int per_wave) { return sizeof(R) =
 // where it is used instead of being carried (and rotated) in registers, which leaves the
 // registers to the Riemann solves.  FP32 keeps the window in registers.
 template <int EFLUX, typename R, bool WIDE>
-__device__ __forceinline__ void xy_body(const FusedArgs& a, int w, int lane, char* lds, R* ring, R* edgeP,
+__device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, int lane, char* lds, R* ring, R* edgeP,
                                         double* edgeF, int wv) {
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
   const size_t vs = g.vstride;
   const int iter = sc->iter;
-  const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
+  const double* __restrict__ U = stage == 2 ? a.U1 : (stage == 0 && (iter & 1)) ? a.U1 : a.U0;
   const double chg = bits2d(sc->red[iter % 3][1]);
   const double hchg = 0.5 / chg;
   const int SW = a.sw;
@@ -549,8 +549,9 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
   __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int wv = threadIdx.x >> 6;
-  xy_body<EFLUX, R, WIDE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, smem + wv * xy_wave_lds<R>(a.sw, a.ly),
-                          xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0), xedgeP, xedgeF, wv);
+  xy_body<EFLUX, R, WIDE>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
+                          smem + wv * xy_wave_lds<R>(a.sw, a.ly), xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0),
+                          xedgeP, xedgeF, wv);
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
@@ -561,15 +562,21 @@ constexpr int Z_RAW = 3 * NCONS * 64;
 // one wavefront: an x-strip of sw cells (the xy kernel's strip width) on 64/sw consecutive
 // rows, one column per lane, planes [k0, k1) of chunk zc.  The columns are independent, so
 // narrow boxes stack rows in a wave instead of leaving lanes idle.
-template <int EFLUX, typename R>
+// STAGE: FusedArgs::stage, fixed at compile time so the one-stage kernel carries nothing of
+// the other two.  The second Runge-Kutta stage (STAGE 2) averages each updated cell with the
+// state it overwrites.
+template <int EFLUX, typename R, int STAGE>
 __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
+  constexpr bool AVG = STAGE == 2;
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
   const size_t vs = g.vstride;
   const int iter = sc->iter;
-  const double* __restrict__ U = (iter & 1) ? a.U1 : a.U0;
-  double* __restrict__ Uo = (iter & 1) ? a.U0 : a.U1;
+  const bool odd = STAGE == 2 || (STAGE == 0 && (iter & 1));
+  const double* __restrict__ U = odd ? a.U1 : a.U0;
+  double* __restrict__ Uo = odd ? a.U0 : a.U1;
+  constexpr bool reduce = STAGE != 1;
   const unsigned long long* rd = sc->red[iter % 3];
   const double dt = a.cfl * bits2d(rd[0]);
   const double chg = bits2d(rd[1]);
@@ -662,6 +669,9 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     // when not updated), so the number of loads in flight is the same on every path and a
     // wait for an earlier load need not drain these
     load_acc(a.acc, g.cell(kk, j, i), ac);
+    // the second stage's U^n of cell kk, loaded under the same rule as the divergence
+    double un0[AVG ? NCONS : 1];
+    if constexpr (AVG) load_raw(Uoc, g.cell(kk, j, i), un0);
     R up[NPRIM], dn[NPRIM], upprev[NPRIM];
     if constexpr (PRING) {
       R Pm[NPRIM], P0[NPRIM], Pp[NPRIM];
@@ -704,13 +714,19 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
       const double dhl = fmin(dxa, fmin(dya, dza));
       const double taui = a.alphabp * chg / dhl;
       u[mbps] = u[mbps] * (1.0 - dt * taui);
+      if constexpr (AVG) {
+#pragma unroll
+        for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (un0[m] + u[m]);
+      }
       store_cols(Uoc, oc, u);
       if (img_xy || kk <= NGH + 1 || kk >= NGH + g.nz - 2) write_images(Uo, g, a.bc, i, j, kk, u);
       // the next iteration's ControlTimestep / EvaluateCh bounds of this cell
-      double q[NPRIM];
-      cons_to_prim(u, a.gam, q);
-      dtl = fmin(dtl, cell_dt(q, dxa, dya, dza));
-      chl = fmax(chl, cell_ch(q));
+      if constexpr (reduce) {
+        double q[NPRIM];
+        cons_to_prim(u, a.gam, q);
+        dtl = fmin(dtl, cell_dt(q, dxa, dya, dza));
+        chl = fmax(chl, cell_ch(q));
+      }
     }
     const int t0 = sm;
     sm = s0;
@@ -719,6 +735,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 #pragma unroll
     for (int m = 0; m < NCONS; ++m) Fzprev[m] = Fz[m];
   }
+  if constexpr (!reduce) return;
   const bool bad = !(dtl > 0.0) || !(chl == chl) || isinf(chl);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -736,23 +753,23 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   }
 }
 
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, int STAGE>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
   // LDS: FP64 window ring (96 KiB) + raw ring (60 KiB) per block; FP32 raw ring only
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
-                   zraw + wv * Z_RAW);
+  z_body<EFLUX, R, STAGE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
+                          zraw + wv * Z_RAW);
 }
 
 // ---------------------------------------------------------------- batched launches
 // Many boxes in one launch: tab[block] = (box, block index within that box).  Every box's
 // block count is a whole number of blocks, so a block never straddles two boxes and the
-// box descriptor read is uniform over the block.
+// box descriptor read is uniform over the block.  stage: FusedArgs::stage for every box.
 template <int EFLUX, typename R>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
-                                                                     const int2* __restrict__ tab) {
+                                                                     const int2* __restrict__ tab, int stage) {
   extern __shared__ __align__(16) char smem[];
   __shared__ R xring[xy_ring_len<R>()];
   __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
@@ -762,11 +779,11 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
   const int wv = threadIdx.x >> 6;
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
   R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
-  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
-  else xy_body<EFLUX, R, false>(a, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+  else xy_body<EFLUX, R, false>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
 }
 
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, int STAGE>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const FusedArgs* __restrict__ boxes,
                                                                     const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
@@ -774,8 +791,8 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
-                   zraw + wv * Z_RAW);
+  z_body<EFLUX, R, STAGE>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
+                          zraw + wv * Z_RAW);
 }
 
 // step end of every live box (k_step_end with the fused ring offset), one thread per box
@@ -811,7 +828,10 @@ void launch_xy_t(const FusedArgs& a, hipStream_t s) {
 
 template <int EFLUX, typename R>
 void launch_z_t(const FusedArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fused_z<EFLUX, R>), dim3(fused_blocks_z(a)), dim3(64 * WAVES), 0, s, a);
+  const dim3 grid(fused_blocks_z(a)), block(64 * WAVES);
+  if (a.stage == 1) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 1>), grid, block, 0, s, a);
+  else if (a.stage == 2) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 2>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_fused_z<EFLUX, R, 0>), grid, block, 0, s, a);
 }
 
 struct BatchLaunch {
@@ -819,15 +839,19 @@ struct BatchLaunch {
   const int2* tab;
   int nblocks;
   size_t lds;
+  int stage;
 };
 
 template <int EFLUX, typename R>
 void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab);
+  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab, b.stage);
 }
 template <int EFLUX, typename R>
 void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_z<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), 0, s, b.boxes, b.tab);
+  const dim3 grid(b.nblocks), block(64 * WAVES);
+  if (b.stage == 1) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 1>), grid, block, 0, s, b.boxes, b.tab);
+  else if (b.stage == 2) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 2>), grid, block, 0, s, b.boxes, b.tab);
+  else hipLaunchKernelGGL((k_batch_z<EFLUX, R, 0>), grid, block, 0, s, b.boxes, b.tab);
 }
 
 #define MHD_DISPATCH2(fn, eflux, mixed, a, s)                 \
@@ -902,11 +926,12 @@ int fused_blocks_z(const FusedArgs& a) {
 }
 
 void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, size_t lds, int eflux, int mixed,
-                     hipStream_t s) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, lds}), s);
+                     hipStream_t s, int stage) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, lds, stage}), s);
 }
-void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, 0}), s);
+void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s,
+                    int stage) {
+  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, 0, stage}), s);
 }
 void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s) {
   if (nlive > 0)

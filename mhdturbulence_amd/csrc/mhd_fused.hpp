@@ -23,7 +23,7 @@ struct FusedArgs {
   const double* x1a;
   const double* x2a;
   const double* x3a;
-  double* U0;      // iteration iter reads U[iter & 1] and writes the other copy
+  double* U0;      // stage 0: iteration iter reads U[iter & 1] and writes the other copy
   double* U1;
   double* acc;     // NCONS * vstride, cell-major: acc[c * NCONS + m]
   StepScalars* sc;
@@ -33,6 +33,11 @@ struct FusedArgs {
   int ns;          // strips per x-row
   int ly, lz;      // y-chunk and z-chunk lengths
   int eflux, mixed;
+  // 0: one forward-Euler stage per iteration (the reference's step);
+  // 1 / 2: the first / second stage of the two-stage Runge-Kutta step, U0 -> U1 and then
+  // U1 -> U0 averaged with the U0 it replaces; the state is in U0 after every iteration, and
+  // only the second stage reduces the next iteration's dt and c_h
+  int stage;
 };
 
 // strip width and chunk lengths for a geometry (a.g and a.mixed set), for a box run alone or
@@ -59,10 +64,12 @@ void launch_fused_z(const FusedArgs& a, hipStream_t s);
 
 // Batched launches: boxes[] are device-resident descriptors, tab[block] = (box, block of
 // that box); nblocks = tab length.  All boxes of a batch share eflux and precision.
-// lds: dynamic LDS bytes of the xy launch, the largest fused_xy_lds over its boxes
+// lds: dynamic LDS bytes of the xy launch, the largest fused_xy_lds over its boxes.
+// stage: FusedArgs::stage of every box in the launch (the descriptors' own field is unused)
 void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, size_t lds, int eflux, int mixed,
-                     hipStream_t s);
-void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s);
+                     hipStream_t s, int stage = 0);
+void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s,
+                    int stage = 0);
 // time += dt etc. for each listed box (live[] holds box indices); dt_last[box] gets its dt
 void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s);
 

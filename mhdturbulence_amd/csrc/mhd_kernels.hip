@@ -308,7 +308,8 @@ __global__ __launch_bounds__(BLK) void k_update(Geom g, const double* __restrict
                                                 const double* __restrict__ x2a, const double* __restrict__ x3a,
                                                 const double* __restrict__ Fx, const double* __restrict__ Fy,
                                                 const double* __restrict__ Fz, double* __restrict__ U, double dt,
-                                                double cfl, double alphabp, const StepScalars* sc) {
+                                                double cfl, double alphabp, const StepScalars* sc,
+                                                double* __restrict__ save, const double* __restrict__ avg) {
   const int i = NGH + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = NGH + blockIdx.y * (BLK / 64) + (threadIdx.x >> 6);
   const int k = NGH + blockIdx.z;
@@ -328,12 +329,14 @@ __global__ __launch_bounds__(BLK) void k_update(Geom g, const double* __restrict
   for (int m = 0; m < NCONS; ++m) {
     const size_t o = m * g.vstride + c;
     double u = U[o];
+    if (save) save[o] = u;
     u -= dt * ((Fx[o + 1] - Fx[o]) * xinv + (Fy[o + sy] - Fy[o]) * yinv + (Fz[o + sz] - Fz[o]) * zinv);
     if (m == mbps && sc) {
       const double dhl = fmin(dxa, fmin(dya, dza));
       const double taui = alphabp * chg / dhl;
       u = u * (1.0 - dt * taui);
     }
+    if (avg) u = 0.5 * (avg[o] + u);
     U[o] = u;
   }
 }
@@ -509,9 +512,10 @@ void launch_flux(const Geom& g, int dir, int eflux, int mixed, const double* P, 
 
 void launch_update(const Geom& g, const double* x1a, const double* x2a, const double* x3a, const double* Fx,
                    const double* Fy, const double* Fz, double* U, double dt, double cfl, double alphabp,
-                   const StepScalars* sc, hipStream_t s) {
+                   const StepScalars* sc, hipStream_t s, double* save, const double* avg) {
   dim3 grid(cdiv(g.nx, 64), cdiv(g.ny, BLK / 64), g.nz);
-  hipLaunchKernelGGL(k_update, grid, dim3(BLK), 0, s, g, x1a, x2a, x3a, Fx, Fy, Fz, U, dt, cfl, alphabp, sc);
+  hipLaunchKernelGGL(k_update, grid, dim3(BLK), 0, s, g, x1a, x2a, x3a, Fx, Fy, Fz, U, dt, cfl, alphabp, sc, save,
+                     avg);
 }
 
 void launch_damp(const Geom& g, const double* x1a, const double* x2a, const double* x3a, double* U, double dt,

@@ -38,6 +38,7 @@ class Config:
     grid_shift: int = 1      # 1: C++ variants' face formula; 0: unshifted
     precision: int = 0       # 0: FP64; 1: FP32 reconstruction + HLLD
     device: int = 0
+    stages: int = 1          # 1: forward Euler (the reference); 2: two-stage SSP Runge-Kutta
 
     def to_c(self) -> capi.MhdConfig:
         c = capi.MhdConfig()
@@ -49,6 +50,7 @@ class Config:
         c.gam, c.cfl, c.alphabp = self.gam, self.cfl, self.alphabp
         c.energy_flux, c.grid_shift, c.precision, c.device = (self.energy_flux, self.grid_shift,
                                                               self.precision, self.device)
+        c.stages = self.stages
         return c
 
 
