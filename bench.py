@@ -1,13 +1,16 @@
 """Benchmark: cell-updates/s of the MHD time step on one MI355X, with its HBM roofline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256 | --shape NX NY NZ]
-                    [--mode 0|1] [--precision 0|1] [--no-cpu] [--no-serve]
+                    [--mode 0|1] [--precision 0|1] [--stages 1|2 [--cfl C]] [--no-cpu]
+                    [--no-serve] [--no-rk2]
 
 Workload: a 256^3 FP64 Kelvin-Helmholtz shear-layer box (gamma 5/3, Bx = 0.1,
 vx = 0.5 tanh(y/0.05), seeded vy kick; periodic x,z, reflecting y), advanced with the
 reference's loop body (MUSCL + van Leer, HLLD, GLM, one forward-Euler stage, CFL
 factor 0.05).  The state is generated on the device, so inputs are resident in HBM
 before the timed region.  One step = one loop body over all 256^3 interior cells.
+--stages 2 runs the two-stage Runge-Kutta step instead (CFL factor 0.3 unless --cfl is
+given); one step is then both stages.
 
 Printed JSON line: value = interior cells x steps / wall seconds of the timed region.  In
 the timed region the step's kernels are launched one by one on the library's stream with a
@@ -24,6 +27,9 @@ never part of `value`.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
 host's cores on 16 steps (about 10 s) of the same 256^3 box.
 ensemble / serving: the batched and request-serving measurements (skipped by --no-serve).
+rk2: the same box advanced by the two-stage Runge-Kutta step at CFL factor 0.3 (the
+configuration with "2-stage RK, CFL 0.3"), beside the reference's one-stage step; both
+report simulated time per wall second (skipped by --no-rk2).
 """
 from __future__ import annotations
 
@@ -58,11 +64,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_box(shape, mode_precision: int):
+def make_box(shape, mode_precision: int, stages: int = 1, cfl: float = 0.05):
     from mhdturbulence_amd import Box, Config
     nx, ny, nz = shape
     cfg = Config(nx=nx, ny=ny, nz=nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
-                 gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0, precision=mode_precision)
+                 gam=5.0 / 3.0, cfl=cfl, energy_flux=1, grid_shift=0, precision=mode_precision, stages=stages)
     b = Box(cfg)
     b.generate_grid()
     b.generate_problem_shear(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=0.1, amp=1e-2, sigma=0.1, seed=20261203)
@@ -165,6 +171,27 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203) -> di
     return out
 
 
+def rk2_bench(shape, precision: int, mode: int, euler_sim_rate: float, steps: int = 60) -> dict:
+    """the two-stage Runge-Kutta step (CFL factor 0.3) on the same box: cell-updates/s (one
+    update = both stages), per-kernel ms (both stages of a kernel summed) and simulated time
+    per wall second against the one-stage step's"""
+    cells = shape[0] * shape[1] * shape[2]
+    box = make_box(shape, precision, 2, 0.3)
+    t, _, _ = box.run(5, mode=mode)
+    prof = box.profile_kernels(10, mode=mode)
+    t, _, _ = box.run(2, time=t, mode=mode)
+    t_before = t
+    t0 = time.perf_counter()
+    t, _, done = box.run(steps, time=t, mode=mode)
+    wall = time.perf_counter() - t0
+    out = {"stages": 2, "cfl": 0.3, "steps": done, "cell_updates_per_s": cells * done / wall,
+           "ms_per_step": 1e3 * wall / max(done, 1), "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
+           "sim_time_per_wall_s": (t - t_before) / wall, "status": box.status()}
+    out["sim_time_rate_vs_one_stage"] = out["sim_time_per_wall_s"] / euler_sim_rate
+    box.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -181,7 +208,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--shape", type=int, nargs=3, metavar=("NX", "NY", "NZ"), default=None,
                     help="a non-cubic box instead of n^3 (e.g. one GPU's z-slab of a sharded box)")
+    ap.add_argument("--stages", type=int, default=1, choices=(1, 2),
+                    help="1: the reference's forward-Euler step; 2: two-stage Runge-Kutta")
+    ap.add_argument("--cfl", type=float, default=None, help="CFL factor (default 0.05, or 0.3 with --stages 2)")
+    ap.add_argument("--no-rk2", action="store_true")
     args = ap.parse_args()
+    cfl = args.cfl if args.cfl is not None else (0.3 if args.stages == 2 else 0.05)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
               file=sys.stderr)
@@ -193,7 +225,7 @@ def main():
     cube = shape[0] == shape[1] == shape[2]
     shape_name = f"{n}^3" if cube and not args.shape else "x".join(map(str, shape))
     log(f"[bench] building {shape_name} box on the device")
-    box = make_box(shape, args.precision)
+    box = make_box(shape, args.precision, args.stages, cfl)
     log(f"[bench] device bytes {box.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
     t = 0.0
     t, _, _ = box.run(args.warmup, time=t, mode=args.mode)
@@ -206,10 +238,13 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     log(f"[bench] {value:.4e} cell-updates/s, {ms_per_step:.3f} ms/step (kernels {sum(prof.values()):.3f} ms/step)")
     # the production loop (each step a replayed graph, no events) over --profile-steps steps
+    t_before = t
     t0 = time.perf_counter()
     t, _, done = box.run(args.profile_steps, time=t, mode=args.mode)
-    graph_ms = 1e3 * (time.perf_counter() - t0) / max(done, 1)
-    log(f"[bench] graph replay {graph_ms:.3f} ms/step")
+    graph_wall = time.perf_counter() - t0
+    graph_ms = 1e3 * graph_wall / max(done, 1)
+    sim_rate = (t - t_before) / graph_wall
+    log(f"[bench] graph replay {graph_ms:.3f} ms/step, {sim_rate:.4f} simulated time units per wall second")
     # host buffers across the boundary (not part of `value`): the reference's FieldArray
     # h2d of P and U and the d2h of P, timed once each after a warm-up transfer, into host
     # arrays allocated once and already touched (as the reference's P and U are)
@@ -226,16 +261,19 @@ def main():
             "cell_updates_per_s_if_each_step_round_trips": cells / ((ms_per_step + up_ms + down_ms) * 1e-3)}
     log(f"[bench] host transfers: upload P+U {up_ms:.1f} ms, download P {down_ms:.1f} ms")
     dominant = max(prof, key=prof.get)
-    dom_ms = prof[dominant]
-    step_ms = sum(prof.values())
+    # per-kernel times sum both stages of a two-stage step; the roofline is per launch
+    dom_ms = prof[dominant] / (1 if dominant in ("step_end", "reduce_dt_ch") else args.stages)
+    step_ms = sum(prof.values()) / args.stages
     # algorithmic bytes of one launch of the dominant kernel: the per-cell-update figure (160 B,
     # read + write of the FP64 conserved state) times the cells that launch processes (every
-    # interior cell); "step" relates the same bytes to the whole step's kernel time
+    # interior cell); "step" relates the same bytes to the whole step's kernel time (per
+    # stage for a two-stage step)
     alg_bytes = BYTES_PER_CELL * cells
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": MEASURED_TRAFFIC.get((shape[0], args.mode, args.precision, dominant)) if cube else None,
+                "traffic": (MEASURED_TRAFFIC.get((shape[0], args.mode, args.precision, dominant))
+                            if cube and args.stages == 1 else None),
                 "kernel": dominant,
                 "kernel_ms": round(dom_ms, 4),
                 "step": {"achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2),
@@ -247,16 +285,20 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f64 state, f32 reconstruction+HLLD",
         "data": "synthetic (device-generated seeded shear-layer state)",
         "config": {"workload": f"KH shear layer {shape_name}, {'fused' if args.mode == 0 else 'staged'} step kernels",
-                   "cells": cells, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": 0.05, "stages": 1,
+                   "cells": cells, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": cfl, "stages": args.stages,
                    "energy_flux": "(v.B)B_n", "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
         "graph_replay_ms_per_step": round(graph_ms, 4),
+        "sim_time_per_wall_s": sim_rate,
         "host_transfer": pcie,
         "status": status,
         "sim_time": t,
     }
     del box
+    if not args.no_rk2 and args.stages == 1:
+        log(f"[bench] two-stage Runge-Kutta step at CFL 0.3, {shape_name}")
+        out["rk2"] = rk2_bench(shape, args.precision, args.mode, sim_rate)
     if not args.no_serve:
         log("[bench] batched ensemble: 96 boxes, batched FP64 / mixed vs one at a time")
         out["ensemble"] = ensemble_bench()
