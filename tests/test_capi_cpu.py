@@ -48,6 +48,17 @@ def test_bad_config_rejected_before_device_use():
     assert L.mhd_create(C.byref(cfg), C.byref(h)) == -1
     assert L.mhd_create(None, C.byref(h)) == -1
     assert L.mhd_create(C.byref(cfg), None) == -1
+    # stages: 0 or 1 (forward Euler) or 2 (two-stage Runge-Kutta)
+    cfg.bc[2] = 2
+    cfg.gam, cfg.cfl = 1.4, 0.05
+    for q in range(3):
+        cfg.xmin[q], cfg.xmax[q] = 0.0, 1.0
+    for bad in (3, -1):
+        cfg.stages = bad
+        assert L.mhd_create(C.byref(cfg), C.byref(h)) == -1 and not h.value
+        assert b"stages" in L.mhd_last_error()
+    assert L.mhd_batch_create(C.byref(cfg), 2, 1 << 20, C.byref(h)) == -1
+    assert b"stages" in L.mhd_last_error()
 
 
 def test_null_context_calls_fail_cleanly():
