@@ -124,7 +124,9 @@ int mhd_update_primitive(mhd_ctx* ctx);
  * tmax > 0 the loop stops after the step that takes time past tmax (main.cpp:265).
  * dt_log (may be NULL, capacity nsteps) receives each dt.  *steps_done (may be NULL)
  * returns the steps applied.  P and U are both valid afterwards.
- * mode: 0 = fused step kernel (default), 1 = staged kernels (one per reference function). */
+ * mode: 0 = fused step kernel (default), 1 = staged kernels (one per reference function).
+ * With mhd_config.stages = 2 each iteration is the two-stage Runge-Kutta step over that
+ * loop body's update (one dt and c_h per iteration, from its starting state). */
 int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done);
 
 /* Kelvin-Helmholtz initial state of src_cpp_kokkos_host/main.cpp:66-184 on the device
