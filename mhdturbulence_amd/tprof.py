@@ -12,6 +12,10 @@ The growth rate of sqrt(<vy^2>) is fitted (least squares of its logarithm) over
 both are compared with Gamma.
 
     python -m mhdturbulence_amd.tprof [--n 150] [--t-end 15] [--interval 0.1] [--out DIR]
+                                      [--stages 1|2] [--cfl C]
+
+--stages 2 advances with the two-stage Runge-Kutta step instead of the reference's
+forward-Euler step (CFL factor 0.3 unless --cfl is given).
 
 Prints one JSON object (wall time, steps, seconds per cell per step, fitted rate).
 """
@@ -31,8 +35,11 @@ A_REF, GAMMA_REF = 1.2e-3, 1.49
 
 
 def run(n: int, t_end: float, interval: float, out_dir: str | None, fit=(0.5, 3.0), precision: int = 0,
-        log=None) -> dict:
-    box = Box(Config(nx=n, ny=n, nz=n, precision=precision))  # the KH configuration
+        log=None, stages: int = 1, cfl: float | None = None) -> dict:
+    if cfl is None:
+        cfl = 0.3 if stages == 2 else 0.05
+    # the KH configuration
+    box = Box(Config(nx=n, ny=n, nz=n, precision=precision, stages=stages, cfl=cfl))
     box.generate_grid()
     box.generate_problem_kh()
     rows = []
@@ -79,7 +86,8 @@ def run(n: int, t_end: float, interval: float, out_dir: str | None, fit=(0.5, 3.
             g = float(np.polyfit(arr[w, 0], np.log(arr[w, 2]), 1)[0])
             if not peak >= g:
                 peak, t_peak = g, float(t_a)
-    return {"n": n, "t_end": t_end, "time_reached": t, "steps": steps, "status": status, "wall_s": wall,
+    return {"n": n, "t_end": t_end, "stages": stages, "cfl": cfl, "time_reached": t, "steps": steps,
+            "status": status, "wall_s": wall,
             "s_per_cell_step": wall / (n ** 3 * max(steps, 1)), "fit_window": list(fit), "growth_rate": gamma,
             "peak_growth_rate": peak, "peak_window_start": t_peak, "reference_growth_rate": GAMMA_REF,
             "vy_rms_peak": float(arr[:, 2].max()), "t_of_peak": float(arr[int(arr[:, 2].argmax()), 0]),
@@ -94,12 +102,14 @@ def main(argv=None):
     ap.add_argument("--fit", type=float, nargs=2, default=[0.5, 3.0])
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--out", default=None, help="directory for t-prof.csv")
+    ap.add_argument("--stages", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--cfl", type=float, default=None)
     a = ap.parse_args(argv)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    print(json.dumps(run(a.n, a.t_end, a.interval, a.out, tuple(a.fit), a.precision, log)))
+    print(json.dumps(run(a.n, a.t_end, a.interval, a.out, tuple(a.fit), a.precision, log, a.stages, a.cfl)))
 
 
 if __name__ == "__main__":
