@@ -142,3 +142,25 @@ def test_rk2_batch_matches_oracle_and_single_box(orc):
     bt2.step(15)
     _, Pb = bt2.finish_box(bid, want_prim=True)
     assert parity.rel_l1(x.interior(Pb), x.interior(x.download_prim()), "prim").max() < 1e-13
+
+
+def test_rk2_mixed_batch_equals_single_box():
+    """mixed precision, two stages: a box in a batch (beside boxes of other sizes) matches the
+    same box stepped alone by the single-box path.  The two run different chunk lengths, and
+    FP32 faces computed in different code paths may contract differently, so the bound is
+    FP32 round-off over the steps rather than bitwise"""
+    s, ic = (48, 32, 16), dict(a=0.08, bx0=0.1, amp=0.005, p0=2.5, seed=12)
+    cfg = Config(**BASE, precision=1)
+    x = Box(Config(nx=s[0], ny=s[1], nz=s[2], **BASE, precision=1))
+    x.generate_grid()
+    x.generate_problem_shear(**ic)
+    x.run(12, mode=0)
+    bt = Batch(cfg, 4, 2 << 30)
+    bt.add_box(32, 32, 32, ic=dict(a=0.05, seed=11))
+    bid = bt.add_box(*s, ic=ic)
+    bt.add_box(24, 40, 32, ic=dict(a=0.03, seed=13))
+    bt.step(12)
+    _, Pb = bt.finish_box(bid, want_prim=True)
+    err = parity.rel_l1(x.interior(Pb), x.interior(x.download_prim()), "prim").max()
+    say(f"rk2 mixed batch vs single box: max relL1 {err:.3e}")
+    assert err < 1e-8
