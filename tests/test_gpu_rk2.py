@@ -80,6 +80,20 @@ def test_rk2_chunks_and_ragged_boxes(orc, shape, tiling):
         assert np.all(errP <= TOL_L1) and np.all(errU <= TOL_L1), (mode, errP, errU)
 
 
+@pytest.mark.parametrize("bc", [(3, 3, 2, 2, 3, 3), (2, 2, 3, 3, 1, 1)])
+def test_rk2_other_boundaries(orc, bc):
+    """outflow and reflecting walls: the ghost images of each stage's output (stage 2: of the
+    averaged state) are what the next stage and step read"""
+    say("rk2 boundaries", bc)
+    for mode in (0, 1):
+        o, b = _pair(orc, (24, 24, 20), bc=bc)
+        o.run(25)
+        _, _, done = b.run(25, mode=mode)
+        assert done == 25
+        errP, errU = _errs(o, b)
+        assert np.all(errP <= TOL_L1) and np.all(errU <= TOL_L1), (mode, errP, errU)
+
+
 def test_rk2_run_continues_across_calls_and_stops_at_tmax(orc):
     o, b = _pair(orc, (24, 24, 16))
     t1, _, d1 = b.run(7, mode=0)
