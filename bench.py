@@ -53,8 +53,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1358522.3 * 1024 * 2.0 + 1310722.0 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1374025.0 * 1024 * 2.0 + 1392898.1 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1356751.4 * 1024 * 2.0 + 1310725.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1374071.2 * 1024 * 2.0 + 1392898.2 * 1024 * 1.0,
     (256, 0, 1, "fused_xy"): 732331.7 * 1024 * 2.0 + 1310735.2 * 1024 * 1.0,
     (256, 0, 1, "fused_z_update"): 1559755.6 * 1024 * 2.0 + 1393848.5 * 1024 * 1.0,
 }

@@ -230,6 +230,9 @@ constexpr int WIN_RING = 4 * NPRIM * 64;
 #ifndef MHD_XY_COOP
 #define MHD_XY_COOP 1
 #endif
+#ifndef MHD_XY_SHARE
+#define MHD_XY_SHARE 1
+#endif
 // block-shared strip-edge exchange of the xy kernels, per wave: edge cells of 4 rows (2 sides
 // each) and the right interface state of the last cell, in values of R; one face flux
 constexpr int XY_EDGE_R = 9 * NPRIM;
@@ -277,6 +280,21 @@ __host__ __device__ inline bool xy_coop(const FusedArgs& a) {
   if (a.ns == 1 && !(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return false;
   return WAVES % a.ns == 0 && a.g.nz % (WAVES / a.ns) == 0;
 }
+
+// Shared pre-pass (WIDE FP64 strips, four strips of one x-row per block, MHD_XY_SHARE=1):
+// the block computes each strip-edge face of its chunk once, 4 or 5 faces per row (4 when
+// the block holds the whole periodic x-row), instead of every wave computing both of its
+// own edges, and one block barrier publishes them.  Per face and row it keeps the flux and
+// both neighbour cells, in the block's slots of the per-wave layout:
+//   F[(fi * ly + r) * NCONS], L[(fi * ly + r) * NPRIM] (cell fc-1), R[...] (cell fc),
+// fi = strip position of the face in the block (the left face of strip fi; fi = 4 the right
+// face of the last strip).
+template <typename R, bool WIDE>
+__host__ __device__ inline bool xy_share(const FusedArgs& a) {
+  return WIDE && MHD_XY_SHARE && sizeof(R) == 8 && a.ns % WAVES == 0;
+}
+static_assert((WAVES + 1) * (NCONS + 2 * NPRIM) * sizeof(double) <= WAVES * 2 * (NCONS + NPRIM) * sizeof(double),
+              "the shared pre-pass slots fit the block's per-wave slots");
 // static LDS of the edge exchange (FP32 only)
 template <typename R>
 __host__ __device__ constexpr int xy_edge_len(int per_wave) { return sizeof(R) == 4 ? WAVES * per_wave : 1; }
@@ -343,10 +361,60 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   auto edge_cell = [&](int wave, int jr, int side) { return edgeP + wave * XY_EDGE_R + ((jr & 3) * 2 + side) * NPRIM; };
   R* const edge_rt = edgeP + wv * XY_EDGE_R + 8 * NPRIM;
 
+  const bool share = xy_share<R, WIDE>(a);
+  // the pre-pass slots this wave's march reads, per row r: left / right face flux and the
+  // outside neighbour cells
+  const double* fL;
+  const double* fR;
+  const R* nL;
+  const R* nR;
+  if (share) {
+    // the block's slots start at this wave's minus wv per-wave blocks
+    char* blk = lds - (size_t)wv * xy_wave_lds<R>(SW, a.ly);
+    double* shF = reinterpret_cast<double*>(blk);
+    R* shL = reinterpret_cast<R*>(blk + (size_t)(WAVES + 1) * a.ly * NCONS * sizeof(double));
+    R* shR = shL + (size_t)(WAVES + 1) * a.ly * NPRIM;
+    const bool wrap4 = a.ns == WAVES && a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC;
+    const int nface = wrap4 ? WAVES : WAVES + 1;
+    const int items = nface * nrows;
+    const int s_first = s - pos;
+    for (int f = wv * 64 + lane; f < items; f += 64 * WAVES) {
+      const int r = f % nrows, fi = f / nrows;
+      const int fc = NGH + (s_first + fi) * 64;  // the face between cells fc-1 and fc
+      R q[4][NPRIM];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) load_prim<R>(U, vs, g.cell(k, j0 + r, fc - 2 + c), a.gam, q[c]);
+      R ql[NPRIM], qr[NPRIM], unused[NPRIM];
+      cell_states<R>(q[0], q[1], q[2], ql, unused);
+      cell_states<R>(q[1], q[2], q[3], unused, qr);
+      double F[NCONS];
+      riemann_flux<0, EFLUX, R>(ql, qr, chg, hchg, F);
+      const int slot = fi * a.ly + r;
+#pragma unroll
+      for (int m = 0; m < NCONS; ++m) shF[slot * NCONS + m] = F[m];
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) {
+        shL[slot * NPRIM + n] = q[1][n];
+        shR[slot * NPRIM + n] = q[2][n];
+      }
+    }
+    const int rfi = (wrap4 && pos == WAVES - 1) ? 0 : pos + 1;
+    fL = shF + (size_t)pos * a.ly * NCONS;
+    fR = shF + (size_t)rfi * a.ly * NCONS;
+    nL = shL + (size_t)pos * a.ly * NPRIM;
+    nR = shR + (size_t)rfi * a.ly * NPRIM;
+    __syncthreads();
+  } else {
+    fL = ldsF + (size_t)(sub * 2) * a.ly * NCONS;
+    fR = fL + (size_t)a.ly * NCONS;
+    nL = ldsN + (size_t)(sub * 2) * a.ly * NPRIM;
+    nR = nL + (size_t)a.ly * NPRIM;
+  }
+
   // pre-pass: the strip-edge x faces of every row of the chunk without a block neighbour,
   // one face per lane
   const int side0 = coop_l ? 1 : 0;
-  const int nsides = 2 - (coop_l ? 1 : 0) - (coop_r ? 1 : 0);
+  const int nsides = share ? 0 : 2 - (coop_l ? 1 : 0) - (coop_r ? 1 : 0);
   const int nf = SPW * nsides * nrows;
   for (int f = lane; f < nf; f += 64) {
     const int r = f % nrows, side = side0 + (f / nrows) % nsides, sb = f / (nsides * nrows);
@@ -415,11 +483,11 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     const bool out_row = row >= j0;
     double ax[NCONS];
     if (out_row) {
-      const int sl = (sub * 2) * a.ly + (row - j0), sr = sl + a.ly;
+      const int rr = row - j0;
       if constexpr (PRING) rdr(sm, Pm);
       R qm[NPRIM], qp[NPRIM];
-      const R* srcl = coop_l ? edge_cell(nb_l, row, 1) : ldsN + sl * NPRIM;
-      const R* srcr = coop_r ? edge_cell(nb_r, row, 0) : ldsN + sr * NPRIM;
+      const R* srcl = coop_l ? edge_cell(nb_l, row, 1) : nL + rr * NPRIM;
+      const R* srcr = coop_r ? edge_cell(nb_r, row, 0) : nR + rr * NPRIM;
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) {
         const R nl = in_reg(srcl[n]);
@@ -461,7 +529,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
       if (!coop_l) {
 #pragma unroll
         for (int m = 0; m < NCONS; ++m) {
-          const double fl = in_reg(ldsF[sl * NCONS + m]);
+          const double fl = in_reg(fL[rr * NCONS + m]);
           Fl[m] = is_left ? fl : Fl[m];
         }
       }
@@ -472,7 +540,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
         }
         __syncthreads();
       }
-      const double* srcf = coop_r ? edgeF + nb_r * NCONS : ldsF + sr * NCONS;
+      const double* srcf = coop_r ? edgeF + nb_r * NCONS : fR + rr * NCONS;
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) {
         const double fe = in_reg(srcf[m]);
