@@ -20,6 +20,9 @@ the production loop (each step a replayed hipGraph, no events) for comparison.
 roofline: algorithmic bytes of a cell update (read + write the 10 FP64 conserved
 variables = 160 B) per launch of the dominant kernel / that kernel's average launch
 duration in the timed region, against 8 TB/s.
+valu: the same launch's VALU lane-instructions (rocprofv3 SQ_INSTS_VALU x 64, measured at
+256^3) over its duration, against every SIMD issuing a wave64 instruction each 4 clocks at
+2.4 GHz; the FP64 kernels run one wavefront per SIMD and are bound by this issue.
 host_transfer: the reference's host-array crossings timed once (upload of P and U, download
 of P, between pageable numpy arrays allocated once), and the rate if every step made that
 round trip;
@@ -58,6 +61,18 @@ MEASURED_TRAFFIC = {
     (256, 0, 1, "fused_xy"): 732331.7 * 1024 * 2.0 + 1310735.2 * 1024 * 1.0,
     (256, 0, 1, "fused_z_update"): 1559755.6 * 1024 * 2.0 + 1393848.5 * 1024 * 1.0,
 }
+
+# VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
+# MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
+MEASURED_VALU = {
+    (256, 0, 0, "fused_xy"): 391.1e6 * 64,
+    (256, 0, 0, "fused_z_update"): 249.2e6 * 64,
+    (256, 0, 1, "fused_xy"): 328.6e6 * 64,
+    (256, 0, 1, "fused_z_update"): 241.5e6 * 64,
+}
+# every SIMD issuing one wave64 VALU instruction per 4 clocks: 256 CUs x 4 SIMDs x 64 lanes
+# / 4 clocks x 2.4 GHz
+VALU_PEAK_LANE_INSTR_S = 256 * 4 * 64 / 4 * 2.4e9
 
 
 def log(*a):
@@ -278,6 +293,13 @@ def main():
                 "kernel_ms": round(dom_ms, 4),
                 "step": {"achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2),
                          "frac": round(alg_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    # the secondary ceiling: VALU issue of the same kernel launch
+    valu_n = MEASURED_VALU.get((shape[0], args.mode, args.precision, dominant)) if cube and args.stages == 1 else None
+    valu = None
+    if valu_n:
+        rate = valu_n / (dom_ms * 1e-3)
+        valu = {"kernel": dominant, "lane_instr_per_launch": valu_n, "achieved": rate,
+                "peak": VALU_PEAK_LANE_INSTR_S, "unit": "lane-instr/s", "frac": round(rate / VALU_PEAK_LANE_INSTR_S, 4)}
     out = {
         "metric": "cell-updates/s (% HBM roofline) at 256^3 FP64",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -288,6 +310,7 @@ def main():
                    "cells": cells, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": cfl, "stages": args.stages,
                    "energy_flux": "(v.B)B_n", "parallelism": "single GPU"},
         "roofline": roofline,
+        "valu": valu,
         "kernels_ms": {k: round(v, 4) for k, v in prof.items()},
         "graph_replay_ms_per_step": round(graph_ms, 4),
         "sim_time_per_wall_s": sim_rate,
