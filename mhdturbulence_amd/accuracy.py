@@ -13,6 +13,10 @@ vx = 0.5 tanh(y/0.05), seeded vy kick, gamma 5/3, CFL factor 0.05):
   mode grows out of it later).
 
     python -m mhdturbulence_amd.accuracy [--n 512 --steps 100] [--growth-n 128 --t-end 3]
+                                         [--stages 2 [--cfl 0.3]]
+
+--stages 2 runs both precisions with the two-stage Runge-Kutta step (CFL factor 0.3 unless
+--cfl is given) instead of the reference's forward-Euler step.
 
 Prints one JSON object.  The FP64 path is the one whose results equal the oracle's (the
 parity tests); this tool measures how far the mixed path departs from it.
@@ -29,22 +33,22 @@ import numpy as np
 from .solver import Box, Config
 
 BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
-            energy_flux=1, grid_shift=0)
+            energy_flux=1, grid_shift=0, stages=1)
 RHO, PRE = 0, 5  # primitive slots of density and pressure
 
 
-def _box(n: int, precision: int, bx0: float) -> Box:
-    b = Box(Config(nx=n, ny=n, nz=n, precision=precision, **BASE))
+def _box(n: int, precision: int, bx0: float, **step) -> Box:
+    b = Box(Config(nx=n, ny=n, nz=n, precision=precision, **{**BASE, **step}))
     b.generate_grid()
     b.generate_problem_shear(rho0=1.0, p0=1.0, v0=0.5, a=0.05, bx0=bx0, amp=1e-2, sigma=0.1, seed=20261203)
     return b
 
 
-def field_errors(n: int, steps: int, bx0: float, log=None) -> dict:
+def field_errors(n: int, steps: int, bx0: float, log=None, **step) -> dict:
     res = {}
     prims = {}
     for prec in (0, 1):
-        b = _box(n, prec, bx0)
+        b = _box(n, prec, bx0, **step)
         t0 = time.perf_counter()
         t, dts, done = b.run(steps)
         wall = time.perf_counter() - t0
@@ -64,10 +68,10 @@ def field_errors(n: int, steps: int, bx0: float, log=None) -> dict:
     return res
 
 
-def growth(n: int, t_end: float, sample_dt: float, bx0: float, fit=(1.5, 3.0), log=None) -> dict:
+def growth(n: int, t_end: float, sample_dt: float, bx0: float, fit=(1.5, 3.0), log=None, **step) -> dict:
     res = {}
     for prec in (0, 1):
-        b = _box(n, prec, bx0)
+        b = _box(n, prec, bx0, **step)
         t = 0.0
         ts, vy = [0.0], [float(b.diagnostics()[2])]
         # steps per sample from the first step's dt (dt changes slowly in the linear phase)
@@ -104,15 +108,19 @@ def main(argv=None):
     ap.add_argument("--sample-dt", type=float, default=0.05)
     ap.add_argument("--fit", type=float, nargs=2, default=[1.5, 3.0])
     ap.add_argument("--bx", type=float, nargs="+", default=[0.05, 0.1, 0.2])
+    ap.add_argument("--stages", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--cfl", type=float, default=None)
     a = ap.parse_args(argv)
+    step = {"stages": a.stages, "cfl": a.cfl if a.cfl is not None else (0.3 if a.stages == 2 else 0.05)}
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    out = {"fields": {"n": a.n, "steps": a.steps}, "growth": {"n": a.growth_n, "t_end": a.t_end, "fit": a.fit}}
+    out = {"step": step, "fields": {"n": a.n, "steps": a.steps},
+           "growth": {"n": a.growth_n, "t_end": a.t_end, "fit": a.fit}}
     for bx in a.bx:
-        out["fields"][f"Bx={bx:g}"] = field_errors(a.n, a.steps, bx, log)
-        g = growth(a.growth_n, a.t_end, a.sample_dt, bx, fit=tuple(a.fit), log=log)
+        out["fields"][f"Bx={bx:g}"] = field_errors(a.n, a.steps, bx, log, **step)
+        g = growth(a.growth_n, a.t_end, a.sample_dt, bx, fit=tuple(a.fit), log=log, **step)
         row = {k: v for k, v in g.items() if k not in ("fp64", "mixed")}
         row.update({f"{p}_growth_rate": g[p]["growth_rate"] for p in ("fp64", "mixed")})
         row["t"] = [round(x, 4) for x in g["fp64"]["t"]]
