@@ -29,7 +29,8 @@ round trip;
 never part of `value`.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
 host's cores on 16 steps (about 10 s) of the same 256^3 box.
-ensemble / serving: the batched and request-serving measurements (skipped by --no-serve).
+ensemble / serving: the batched and request-serving measurements (skipped by --no-serve);
+ensemble.rk2 repeats the ensemble with the two-stage step (skipped by --no-rk2 too).
 rk2: the same box advanced by the two-stage Runge-Kutta step at CFL factor 0.3 (the
 configuration with "2-stage RK, CFL 0.3"), beside the reference's one-stage step; both
 report simulated time per wall second (skipped by --no-rk2).
@@ -149,19 +150,21 @@ def serving_bench(count: int, seed: int = 20261203) -> dict:
     return out
 
 
-def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203) -> dict:
+def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203, stages: int = 1) -> dict:
     """Batched ensemble: `count` boxes, box b drawn by ensemble_box(b) (each axis from
     {32, 48, 64, 96, 128}, a, Bx, amplitude, p from splitmix64(seed + b)), advanced together
     through the batched launches in FP64 and in mixed precision, against the same boxes
     advanced one at a time by the single-box FP64 solver.  Rates are interior cells x steps
-    over the wall time of the timed steps (boxes built and warmed up beforehand)."""
+    over the wall time of the timed steps (boxes built and warmed up beforehand).
+    stages = 2: every solver takes the two-stage Runge-Kutta step at CFL factor 0.3."""
     from mhdturbulence_amd import Batch, Box, Config
     from mhdturbulence_amd.serve import ENSEMBLE_SIZES, ensemble_box
     boxes = [ensemble_box(b, seed) for b in range(count)]
     cells = sum(sh[0] * sh[1] * sh[2] for sh, _ in boxes)
-    out = {"boxes": count, "axis_sizes": list(ENSEMBLE_SIZES), "cells": cells, "steps": steps}
+    base = dict(SHEAR_BASE, stages=stages, cfl=0.3 if stages == 2 else SHEAR_BASE["cfl"])
+    out = {"boxes": count, "axis_sizes": list(ENSEMBLE_SIZES), "cells": cells, "steps": steps, "stages": stages}
     for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
-        bt = Batch(Config(**SHEAR_BASE, precision=prec), max_boxes=count, arena_bytes=48 << 30)
+        bt = Batch(Config(**base, precision=prec), max_boxes=count, arena_bytes=48 << 30)
         for sh, ic in boxes:
             bt.add_box(*sh, ic=ic)
         bt.step(2)
@@ -172,7 +175,7 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203) -> di
         bt.close()
     wall = 0.0
     for sh, ic in boxes:
-        box = Box(Config(nx=sh[0], ny=sh[1], nz=sh[2], **SHEAR_BASE))
+        box = Box(Config(nx=sh[0], ny=sh[1], nz=sh[2], **base))
         box.generate_grid()
         box.generate_problem_shear(**ic)
         box.run(2)
@@ -182,6 +185,8 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203) -> di
         box.close()
     out["fp64_one_at_a_time"] = {"cell_updates_per_s": cells * steps / wall}
     out["batched_fp64_speedup"] = out["batched_fp64"]["cell_updates_per_s"] / out["fp64_one_at_a_time"][
+        "cell_updates_per_s"]
+    out["batched_mixed_speedup"] = out["batched_mixed"]["cell_updates_per_s"] / out["fp64_one_at_a_time"][
         "cell_updates_per_s"]
     return out
 
@@ -325,6 +330,9 @@ def main():
     if not args.no_serve:
         log("[bench] batched ensemble: 96 boxes, batched FP64 / mixed vs one at a time")
         out["ensemble"] = ensemble_bench()
+        if not args.no_rk2:
+            log("[bench] the same ensemble with the two-stage Runge-Kutta step")
+            out["ensemble"]["rk2"] = ensemble_bench(stages=2)
         log(f"[bench] serving: {args.serve_requests} requests, batched vs one at a time")
         out["serving"] = serving_bench(args.serve_requests)
     if not args.no_cpu:
