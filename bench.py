@@ -59,8 +59,8 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     (256, 0, 0, "fused_xy"): 1356751.4 * 1024 * 2.0 + 1310725.0 * 1024 * 1.0,
     (256, 0, 0, "fused_z_update"): 1374071.2 * 1024 * 2.0 + 1392898.2 * 1024 * 1.0,
-    (256, 0, 1, "fused_xy"): 732331.7 * 1024 * 2.0 + 1310735.2 * 1024 * 1.0,
-    (256, 0, 1, "fused_z_update"): 1559755.6 * 1024 * 2.0 + 1393848.5 * 1024 * 1.0,
+    (256, 0, 1, "fused_xy"): 732328.0 * 1024 * 2.0 + 1310725.7 * 1024 * 1.0,
+    (256, 0, 1, "fused_z_update"): 1558325.3 * 1024 * 2.0 + 1393845.1 * 1024 * 1.0,
 }
 
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
