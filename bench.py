@@ -61,6 +61,8 @@ MEASURED_TRAFFIC = {
     (256, 0, 0, "fused_z_update"): 1374071.2 * 1024 * 2.0 + 1392898.2 * 1024 * 1.0,
     (256, 0, 1, "fused_xy"): 732328.0 * 1024 * 2.0 + 1310725.7 * 1024 * 1.0,
     (256, 0, 1, "fused_z_update"): 1558325.3 * 1024 * 2.0 + 1393845.1 * 1024 * 1.0,
+    (256, 0, 2, "fused_xy"): 732336.6 * 1024 * 2.0 + 655380.7 * 1024 * 1.0,
+    (256, 0, 2, "fused_z_update"): 1281360.4 * 1024 * 2.0 + 1393775.9 * 1024 * 1.0,
 }
 
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
@@ -219,7 +221,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--mode", type=int, default=0, help="0 fused step kernels, 1 staged kernels")
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 reconstruction + HLLD")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
+                    help="0 FP64, 1 FP32 reconstruction + HLLD, 2 as 1 with the x-y divergence stored in FP32")
     ap.add_argument("--profile-steps", type=int, default=50,
                     help="steps of the graph-replay run timed after the measured steps")
     ap.add_argument("--cpu-steps", type=int, default=16)
@@ -309,7 +312,8 @@ def main():
         "metric": "cell-updates/s (% HBM roofline) at 256^3 FP64",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f64 state, f32 reconstruction+HLLD",
+        "dtype": {0: "f64", 1: "f64 state, f32 reconstruction+HLLD",
+                  2: "f64 state, f32 reconstruction+HLLD, f32 stored divergence"}[args.precision],
         "data": "synthetic (device-generated seeded shear-layer state)",
         "config": {"workload": f"KH shear layer {shape_name}, {'fused' if args.mode == 0 else 'staged'} step kernels",
                    "cells": cells, "gamma": 5.0 / 3.0, "bx0": 0.1, "cfl": cfl, "stages": args.stages,

@@ -46,7 +46,9 @@ typedef struct mhd_config {
                               1: (E+pt)v_n - (v.B) B_n, as src_cpp_omp_device/mhd.cpp:523-526 */
   int grid_shift;          /* 1: x1a(i) = dx*(i-(ngh+1))+xmin (src_cpp_kokkos_host/main.cpp:39-41);
                               0: dx*(i-ngh)+xmin */
-  int precision;           /* 0: FP64 throughout; 1: reconstruction + HLLD in FP32 */
+  int precision;           /* 0: FP64 throughout; 1: reconstruction + HLLD in FP32;
+                              2: as 1, and the fused step's x-y flux divergence stored in FP32
+                              between its two kernels (the update itself stays FP64) */
   int device;              /* HIP device ordinal */
   int stages;              /* 0 or 1: one forward-Euler stage per step, the reference's
                               (src_cpp_kokkos_host/main.cpp:229-259); 2: the two-stage SSP

@@ -16,7 +16,8 @@ vx = 0.5 tanh(y/0.05), seeded vy kick, gamma 5/3, CFL factor 0.05):
                                          [--stages 2 [--cfl 0.3]]
 
 --stages 2 runs both precisions with the two-stage Runge-Kutta step (CFL factor 0.3 unless
---cfl is given) instead of the reference's forward-Euler step.
+--cfl is given) instead of the reference's forward-Euler step.  --mixed 2 compares the mixed
+mode that also stores the x-y flux divergence in FP32 (precision 2) instead of precision 1.
 
 Prints one JSON object.  The FP64 path is the one whose results equal the oracle's (the
 parity tests); this tool measures how far the mixed path departs from it.
@@ -44,17 +45,17 @@ def _box(n: int, precision: int, bx0: float, **step) -> Box:
     return b
 
 
-def field_errors(n: int, steps: int, bx0: float, log=None, **step) -> dict:
+def field_errors(n: int, steps: int, bx0: float, log=None, mixed: int = 1, **step) -> dict:
     res = {}
     prims = {}
-    for prec in (0, 1):
+    for prec in (0, mixed):
         b = _box(n, prec, bx0, **step)
         t0 = time.perf_counter()
         t, dts, done = b.run(steps)
         wall = time.perf_counter() - t0
         d = b.diagnostics()
         P = b.download_prim()  # (ktot, jtot, 12, itot)
-        prims[prec] = P[2:b.ktot - 3, 2:b.jtot - 3, [RHO, PRE], 2:b.itot - 3]  # interior (nz, ny, 2, nx)
+        prims[min(prec, 1)] = P[2:b.ktot - 3, 2:b.jtot - 3, [RHO, PRE], 2:b.itot - 3]  # interior (nz, ny, 2, nx)
         del P
         res["fp64" if prec == 0 else "mixed"] = {"time": t, "steps": done, "max_divB": float(d[3]),
                                                  "vy_rms": float(d[2]), "wall_s": wall}
@@ -68,9 +69,10 @@ def field_errors(n: int, steps: int, bx0: float, log=None, **step) -> dict:
     return res
 
 
-def growth(n: int, t_end: float, sample_dt: float, bx0: float, fit=(1.5, 3.0), log=None, **step) -> dict:
+def growth(n: int, t_end: float, sample_dt: float, bx0: float, fit=(1.5, 3.0), log=None, mixed: int = 1,
+           **step) -> dict:
     res = {}
-    for prec in (0, 1):
+    for prec in (0, mixed):
         b = _box(n, prec, bx0, **step)
         t = 0.0
         ts, vy = [0.0], [float(b.diagnostics()[2])]
@@ -109,6 +111,7 @@ def main(argv=None):
     ap.add_argument("--fit", type=float, nargs=2, default=[1.5, 3.0])
     ap.add_argument("--bx", type=float, nargs="+", default=[0.05, 0.1, 0.2])
     ap.add_argument("--stages", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--mixed", type=int, default=1, choices=(1, 2), help="the mixed precision level compared")
     ap.add_argument("--cfl", type=float, default=None)
     a = ap.parse_args(argv)
     step = {"stages": a.stages, "cfl": a.cfl if a.cfl is not None else (0.3 if a.stages == 2 else 0.05)}
@@ -116,11 +119,11 @@ def main(argv=None):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    out = {"step": step, "fields": {"n": a.n, "steps": a.steps},
+    out = {"step": step, "mixed_precision": a.mixed, "fields": {"n": a.n, "steps": a.steps},
            "growth": {"n": a.growth_n, "t_end": a.t_end, "fit": a.fit}}
     for bx in a.bx:
-        out["fields"][f"Bx={bx:g}"] = field_errors(a.n, a.steps, bx, log, **step)
-        g = growth(a.growth_n, a.t_end, a.sample_dt, bx, fit=tuple(a.fit), log=log, **step)
+        out["fields"][f"Bx={bx:g}"] = field_errors(a.n, a.steps, bx, log, mixed=a.mixed, **step)
+        g = growth(a.growth_n, a.t_end, a.sample_dt, bx, fit=tuple(a.fit), log=log, mixed=a.mixed, **step)
         row = {k: v for k, v in g.items() if k not in ("fp64", "mixed")}
         row.update({f"{p}_growth_rate": g[p]["growth_rate"] for p in ("fp64", "mixed")})
         row["t"] = [round(x, 4) for x in g["fp64"]["t"]]

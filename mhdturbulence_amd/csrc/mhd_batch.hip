@@ -123,7 +123,8 @@ long long shape_key(int nx, int ny, int nz) { return ((long long)nx << 40) | ((l
 FusedArgs configure(mhd_batch* b, const Geom& g, bool* ok) {
   FusedArgs a{};
   a.g = g;
-  a.mixed = b->base.precision;
+  a.mixed = b->base.precision != 0;
+  a.acc32 = b->base.precision == 2;
   *ok = mhd::fused_configure(a, false);
   auto it = b->tilings.find(shape_key(g.nx, g.ny, g.nz));
   if (it != b->tilings.end()) {
@@ -340,6 +341,7 @@ int mhd_batch_create(const mhd_config* base, int max_boxes, uint64_t arena_bytes
     if (base->bc[q] < 1 || base->bc[q] > 3) return fail(MHD_ERR_ARG, "boundary codes are 1, 2 or 3");
   if (!(base->gam > 1.0) || !(base->cfl > 0.0)) return fail(MHD_ERR_ARG, "gam > 1 and cfl > 0 required");
   if (base->stages < 0 || base->stages > 2) return fail(MHD_ERR_ARG, "stages is 0 or 1 (forward Euler) or 2");
+  if (base->precision < 0 || base->precision > 2) return fail(MHD_ERR_ARG, "precision is 0, 1 or 2");
   int ndev = 0;
   BHIP(hipGetDeviceCount(&ndev));
   if (base->device < 0 || base->device >= ndev) return fail(MHD_ERR_ARG, "no such HIP device");
@@ -465,7 +467,8 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
   a.alphabp = b->base.alphabp;
   for (int q = 0; q < 6; ++q) a.bc[q] = b->base.bc[q];
   a.eflux = b->base.energy_flux;
-  a.mixed = b->base.precision;
+  a.mixed = b->base.precision != 0;
+  a.acc32 = b->base.precision == 2;
   x.fa = a;
   BHIP(hipMemcpyAsync(b->d_args + slot, &a, sizeof a, hipMemcpyHostToDevice, b->stream));
   BCHK(box_prologue(b, x, sc, tmax));

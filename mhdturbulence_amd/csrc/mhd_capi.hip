@@ -111,7 +111,8 @@ int ensure_fused(mhd_ctx* c) {
   if (c->fa_ready) return MHD_OK;
   mhd::FusedArgs& a = c->fa;
   a.g = c->g;
-  a.mixed = c->cfg.precision;
+  a.mixed = c->cfg.precision != 0;
+  a.acc32 = c->cfg.precision == 2;
   if (!mhd::fused_configure(a))
     return fail(MHD_ERR_UNSUPPORTED, "fused step: a variable column of this box spans 4 GiB or more "
                                      "(more than 2^29 cells); use the staged step (mode 1)");
@@ -129,7 +130,8 @@ int ensure_fused(mhd_ctx* c) {
   a.alphabp = c->cfg.alphabp;
   for (int q = 0; q < 6; ++q) a.bc[q] = c->cfg.bc[q];
   a.eflux = c->cfg.energy_flux;
-  a.mixed = c->cfg.precision;
+  a.mixed = c->cfg.precision != 0;
+  a.acc32 = c->cfg.precision == 2;
   c->fa_ready = true;
   return MHD_OK;
 }
@@ -159,7 +161,7 @@ int check_cfg(const mhd_config* cfg) {
   if (!(cfg->gam > 1.0)) return fail(MHD_ERR_ARG, "gam must exceed 1");
   if (!(cfg->cfl > 0.0)) return fail(MHD_ERR_ARG, "cfl must be positive");
   if (cfg->energy_flux < 0 || cfg->energy_flux > 1) return fail(MHD_ERR_ARG, "energy_flux is 0 or 1");
-  if (cfg->precision < 0 || cfg->precision > 1) return fail(MHD_ERR_ARG, "precision is 0 or 1");
+  if (cfg->precision < 0 || cfg->precision > 2) return fail(MHD_ERR_ARG, "precision is 0, 1 or 2");
   if (cfg->stages < 0 || cfg->stages > 2) return fail(MHD_ERR_ARG, "stages is 0 or 1 (forward Euler) or 2");
   return MHD_OK;
 }

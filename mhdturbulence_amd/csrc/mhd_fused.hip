@@ -92,6 +92,17 @@ __device__ __forceinline__ void load_acc(const double* __restrict__ acc, int c, 
   }
 }
 
+// The flux divergence in FP32 (precision 2): variable-major, so a wavefront's 64 cells of
+// one variable are one 256-byte store or load
+__device__ __forceinline__ void store_acc32(float* __restrict__ acc, size_t vs, int c, const double* v) {
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) acc[m * vs + c] = float(v[m]);
+}
+__device__ __forceinline__ void load_acc32(const float* __restrict__ acc, size_t vs, int c, double* v) {
+#pragma unroll
+  for (int m = 0; m < NCONS; ++m) v[m] = double(acc[m * vs + c]);
+}
+
 // conserved state of one cell -> primitive vector in precision R (UpdatePrimitvP arithmetic in FP64)
 template <typename R>
 __device__ __forceinline__ void load_prim(const double* __restrict__ U, size_t vs, int c, double gam, R* q) {
@@ -304,7 +315,9 @@ __host__ __device__ constexpr int xy_edge_len(int per_wave) { return sizeof(R) =
 // rotation and slot 3 the upper state of row j-1.  A row is written once per pass and read
 // where it is used instead of being carried (and rotated) in registers, which leaves the
 // registers to the Riemann solves.  FP32 keeps the window in registers.
-template <int EFLUX, typename R, bool WIDE>
+// ACC32: the divergence is stored in FP32 (FusedArgs::acc32), fixed at compile time so the
+// other kernels carry nothing of it
+template <int EFLUX, typename R, bool WIDE, bool ACC32 = false>
 __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, int lane, char* lds, R* ring, R* edgeP,
                                         double* edgeF, int wv) {
   const StepScalars* sc = a.sc;
@@ -581,7 +594,8 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
       double d[NCONS];
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) d[m] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
-      store_acc(a.acc, g.cell(k, row, i), d);
+      if constexpr (ACC32) store_acc32(reinterpret_cast<float*>(a.acc), vs, g.cell(k, row, i), d);
+      else store_acc(a.acc, g.cell(k, row, i), d);
     }
     if constexpr (PRING) {
       const int t0 = sm;
@@ -610,14 +624,14 @@ __host__ __device__ inline bool xy_wide(const FusedArgs& a) { return MHD_XY_DPP 
 template <typename R>
 __host__ __device__ constexpr int xy_ring_len() { return (MHD_XY_RING && sizeof(R) == 8) ? WAVES * WIN_RING : 1; }
 
-template <int EFLUX, typename R, bool WIDE>
+template <int EFLUX, typename R, bool WIDE, bool ACC32>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ R xring[xy_ring_len<R>()];
   __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
   __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int wv = threadIdx.x >> 6;
-  xy_body<EFLUX, R, WIDE>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
+  xy_body<EFLUX, R, WIDE, ACC32>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
                           smem + wv * xy_wave_lds<R>(a.sw, a.ly), xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0),
                           xedgeP, xedgeF, wv);
 }
@@ -633,7 +647,7 @@ constexpr int Z_RAW = 3 * NCONS * 64;
 // STAGE: FusedArgs::stage, fixed at compile time so the one-stage kernel carries nothing of
 // the other two.  The second Runge-Kutta stage (STAGE 2) averages each updated cell with the
 // state it overwrites.
-template <int EFLUX, typename R, int STAGE>
+template <int EFLUX, typename R, int STAGE, bool ACC32 = false>
 __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
   constexpr bool AVG = STAGE == 2;
   StepScalars* sc = a.sc;
@@ -736,7 +750,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     // issued by every lane on every pass (cell kk >= k0 - 2 >= 0 is in the array; unused
     // when not updated), so the number of loads in flight is the same on every path and a
     // wait for an earlier load need not drain these
-    load_acc(a.acc, g.cell(kk, j, i), ac);
+    if constexpr (ACC32) load_acc32(reinterpret_cast<const float*>(a.acc), vs, g.cell(kk, j, i), ac);
+    else load_acc(a.acc, g.cell(kk, j, i), ac);
     // the second stage's U^n of cell kk, loaded under the same rule as the divergence
     double un0[AVG ? NCONS : 1];
     if constexpr (AVG) load_raw(Uoc, g.cell(kk, j, i), un0);
@@ -821,13 +836,14 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   }
 }
 
-template <int EFLUX, typename R, int STAGE>
+template <int EFLUX, typename R, int STAGE, bool ACC32>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a) {
   // LDS: FP64 window ring (96 KiB) + raw ring (60 KiB) per block; FP32 raw ring only
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R, STAGE>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
+  z_body<EFLUX, R, STAGE, ACC32>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63,
+                                 zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                           zraw + wv * Z_RAW);
 }
 
@@ -835,7 +851,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
 // Many boxes in one launch: tab[block] = (box, block index within that box).  Every box's
 // block count is a whole number of blocks, so a block never straddles two boxes and the
 // box descriptor read is uniform over the block.  stage: FusedArgs::stage for every box.
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, bool ACC32>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
                                                                      const int2* __restrict__ tab, int stage) {
   extern __shared__ __align__(16) char smem[];
@@ -847,11 +863,12 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
   const int wv = threadIdx.x >> 6;
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
   R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
-  if (xy_wide(a)) xy_body<EFLUX, R, true>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
-  else xy_body<EFLUX, R, false>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+  if (xy_wide(a))
+    xy_body<EFLUX, R, true, ACC32>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+  else xy_body<EFLUX, R, false, ACC32>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
 }
 
-template <int EFLUX, typename R, int STAGE>
+template <int EFLUX, typename R, int STAGE, bool ACC32>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const FusedArgs* __restrict__ boxes,
                                                                     const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
@@ -859,7 +876,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
   const int wv = threadIdx.x >> 6;
-  z_body<EFLUX, R, STAGE>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
+  z_body<EFLUX, R, STAGE, ACC32>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                           zraw + wv * Z_RAW);
 }
 
@@ -885,21 +902,22 @@ __global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int*
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, bool ACC32>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
   if (xy_wide(a))
-    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s, a);
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true, ACC32>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
+                       fused_xy_lds(a), s, a);
   else
-    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, false>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), fused_xy_lds(a), s,
-                       a);
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, false, ACC32>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
+                       fused_xy_lds(a), s, a);
 }
 
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, bool ACC32>
 void launch_z_t(const FusedArgs& a, hipStream_t s) {
   const dim3 grid(fused_blocks_z(a)), block(64 * WAVES);
-  if (a.stage == 1) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 1>), grid, block, 0, s, a);
-  else if (a.stage == 2) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 2>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_fused_z<EFLUX, R, 0>), grid, block, 0, s, a);
+  if (a.stage == 1) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 1, ACC32>), grid, block, 0, s, a);
+  else if (a.stage == 2) hipLaunchKernelGGL((k_fused_z<EFLUX, R, 2, ACC32>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_fused_z<EFLUX, R, 0, ACC32>), grid, block, 0, s, a);
 }
 
 struct BatchLaunch {
@@ -910,27 +928,33 @@ struct BatchLaunch {
   int stage;
 };
 
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, bool ACC32>
 void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_xy<EFLUX, R>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab, b.stage);
+  hipLaunchKernelGGL((k_batch_xy<EFLUX, R, ACC32>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab,
+                     b.stage);
 }
-template <int EFLUX, typename R>
+template <int EFLUX, typename R, bool ACC32>
 void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
   const dim3 grid(b.nblocks), block(64 * WAVES);
-  if (b.stage == 1) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 1>), grid, block, 0, s, b.boxes, b.tab);
-  else if (b.stage == 2) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 2>), grid, block, 0, s, b.boxes, b.tab);
-  else hipLaunchKernelGGL((k_batch_z<EFLUX, R, 0>), grid, block, 0, s, b.boxes, b.tab);
+  if (b.stage == 1) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 1, ACC32>), grid, block, 0, s, b.boxes, b.tab);
+  else if (b.stage == 2) hipLaunchKernelGGL((k_batch_z<EFLUX, R, 2, ACC32>), grid, block, 0, s, b.boxes, b.tab);
+  else hipLaunchKernelGGL((k_batch_z<EFLUX, R, 0, ACC32>), grid, block, 0, s, b.boxes, b.tab);
 }
 
-#define MHD_DISPATCH2(fn, eflux, mixed, a, s)                 \
-  do {                                                         \
-    if ((eflux) == 0) {                                        \
-      if (mixed) fn<0, float>(a, s); else fn<0, double>(a, s); \
-    } else {                                                   \
-      if (mixed) fn<1, float>(a, s); else fn<1, double>(a, s); \
-    }                                                          \
+// prec: mhd_config.precision (0 FP64; 1 FP32 reconstruction + HLLD; 2 as 1 with the
+// divergence stored in FP32)
+#define MHD_DISPATCH_E(fn, E, prec, a, s)        \
+  do {                                          \
+    if ((prec) == 2) fn<E, float, true>(a, s);  \
+    else if (prec) fn<E, float, false>(a, s);   \
+    else fn<E, double, false>(a, s);            \
   } while (0)
-#define MHD_DISPATCH(fn, a, s) MHD_DISPATCH2(fn, (a).eflux, (a).mixed, a, s)
+#define MHD_DISPATCH2(fn, eflux, prec, a, s)               \
+  do {                                                     \
+    if ((eflux) == 0) MHD_DISPATCH_E(fn, 0, prec, a, s);   \
+    else MHD_DISPATCH_E(fn, 1, prec, a, s);                \
+  } while (0)
+#define MHD_DISPATCH(fn, a, s) MHD_DISPATCH2(fn, (a).eflux, ((a).mixed ? ((a).acc32 ? 2 : 1) : 0), a, s)
 
 }  // namespace
 

@@ -25,7 +25,7 @@ struct FusedArgs {
   const double* x3a;
   double* U0;      // stage 0: iteration iter reads U[iter & 1] and writes the other copy
   double* U1;
-  double* acc;     // NCONS * vstride, cell-major: acc[c * NCONS + m]
+  double* acc;     // NCONS * vstride, cell-major: acc[c * NCONS + m] (see acc32)
   StepScalars* sc;
   double gam, cfl, alphabp;
   int bc[6];
@@ -33,6 +33,9 @@ struct FusedArgs {
   int ns;          // strips per x-row
   int ly, lz;      // y-chunk and z-chunk lengths
   int eflux, mixed;
+  // FP32 kernels only: 1 = the flux divergence is stored in FP32, variable-major
+  // (((float*)acc)[m * vstride + c]); 0 = FP64, cell-major
+  int acc32;
   // 0: one forward-Euler stage per iteration (the reference's step);
   // 1 / 2: the first / second stage of the two-stage Runge-Kutta step, U0 -> U1 and then
   // U1 -> U0 averaged with the U0 it replaces; the state is in U0 after every iteration, and

@@ -36,7 +36,7 @@ class Config:
     alphabp: float = 0.1
     energy_flux: int = 0     # 0: |B|^2 B_n (src_cpp_kokkos_host); 1: (v.B) B_n
     grid_shift: int = 1      # 1: C++ variants' face formula; 0: unshifted
-    precision: int = 0       # 0: FP64; 1: FP32 reconstruction + HLLD
+    precision: int = 0       # 0: FP64; 1: FP32 reconstruction + HLLD; 2: also the divergence stored in FP32
     device: int = 0
     stages: int = 1          # 1: forward Euler (the reference); 2: two-stage SSP Runge-Kutta
 

@@ -143,3 +143,20 @@ def test_batch_mixed_shared_strip_edges_alone_equals_mix(orc):
         say(s, "mixed vs oracle", err[[0, 5]])
         assert err[[0, 5]].max() < 1e-6 and err.max() < 1e-4, (s, err)
     mix.close()
+
+
+def test_batch_fp32_divergence_matches_single_box():
+    """precision 2 in a batch (chunks of other lengths, boxes of other sizes beside it)
+    against the same box run alone: FP32 round-off apart"""
+    s, ic = SHAPES[1], ICS[1]
+    b = _batch(precision=2)
+    ids = [b.add_box(*sh, ic=c) for sh, c in zip(SHAPES, ICS)]
+    b.step(15)
+    _, Pb = b.finish_box(ids[1], want_prim=True)
+    x = Box(Config(nx=s[0], ny=s[1], nz=s[2], **BASE, precision=2))
+    x.generate_grid()
+    x.generate_problem_shear(**ic)
+    x.run(15, mode=0)
+    err = parity.rel_l1(x.interior(Pb), x.interior(x.download_prim()), "prim")
+    say("precision 2 batch vs alone", err.max())
+    assert err.max() < 1e-8, err

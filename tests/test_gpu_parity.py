@@ -356,10 +356,13 @@ def test_diagnostics_match(orc):
     assert dg[3] == pytest.approx(do[3], rel=1e-6, abs=1e-12)
 
 
-def test_mixed_precision_close_to_fp64(orc):
-    say("mixed precision")
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_close_to_fp64(orc, precision):
+    """precision 1: FP32 reconstruction + HLLD; 2: also the x-y divergence stored in FP32
+    between the fused kernels (the update itself in FP64)"""
+    say("mixed precision", precision)
     o, b = pair(orc, n=(32, 32, 32), problem=dict(bx0=0.1, a=0.05, amp=0.02), energy_flux=1, gam=5.0 / 3.0,
-                precision=1)
+                precision=precision)
     o.run(50)
     b.run(50, mode=0)
     err = rel_l1(o.interior(b.download_prim()), o.interior(o.P))

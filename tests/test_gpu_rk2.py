@@ -178,3 +178,12 @@ def test_rk2_mixed_batch_equals_single_box():
     err = parity.rel_l1(x.interior(Pb), x.interior(x.download_prim()), "prim").max()
     say(f"rk2 mixed batch vs single box: max relL1 {err:.3e}")
     assert err < 1e-8
+
+
+def test_rk2_fp32_divergence_close_to_fp64(orc):
+    o, b = _pair(orc, (32, 32, 32), precision=2)
+    o.run(40)
+    b.run(40, mode=0)
+    errP, _ = _errs(o, b)
+    say(f"rk2 precision 2 vs fp64 oracle: max relL1 P={errP.max():.3e}")
+    assert errP[0] < 1e-6 and errP[5] < 1e-6 and errP.max() < 1e-4, errP
