@@ -155,7 +155,8 @@ def serving_bench(count: int, seed: int = 20261203) -> dict:
 def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203, stages: int = 1) -> dict:
     """Batched ensemble: `count` boxes, box b drawn by ensemble_box(b) (each axis from
     {32, 48, 64, 96, 128}, a, Bx, amplitude, p from splitmix64(seed + b)), advanced together
-    through the batched launches in FP64 and in mixed precision, against the same boxes
+    through the batched launches in FP64 and in mixed precision (precision 1, and 2 with the
+    divergence stored in FP32), against the same boxes
     advanced one at a time by the single-box FP64 solver.  Rates are interior cells x steps
     over the wall time of the timed steps (boxes built and warmed up beforehand).
     stages = 2: every solver takes the two-stage Runge-Kutta step at CFL factor 0.3."""
@@ -165,7 +166,7 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203, stage
     cells = sum(sh[0] * sh[1] * sh[2] for sh, _ in boxes)
     base = dict(SHEAR_BASE, stages=stages, cfl=0.3 if stages == 2 else SHEAR_BASE["cfl"])
     out = {"boxes": count, "axis_sizes": list(ENSEMBLE_SIZES), "cells": cells, "steps": steps, "stages": stages}
-    for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
+    for prec, name in ((0, "batched_fp64"), (1, "batched_mixed"), (2, "batched_mixed_f32_divergence")):
         bt = Batch(Config(**base, precision=prec), max_boxes=count, arena_bytes=48 << 30)
         for sh, ic in boxes:
             bt.add_box(*sh, ic=ic)
@@ -190,6 +191,8 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203, stage
         "cell_updates_per_s"]
     out["batched_mixed_speedup"] = out["batched_mixed"]["cell_updates_per_s"] / out["fp64_one_at_a_time"][
         "cell_updates_per_s"]
+    out["batched_mixed_f32_divergence_speedup"] = out["batched_mixed_f32_divergence"]["cell_updates_per_s"] / out[
+        "fp64_one_at_a_time"]["cell_updates_per_s"]
     return out
 
 

@@ -15,7 +15,8 @@ BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5)
 
 
 @pytest.mark.parametrize("bad", [dict(nx=3), dict(bc=(1, 1, 2, 2, 1, 4)), dict(gam=1.0), dict(cfl=0.0),
-                                 dict(energy_flux=2), dict(precision=2), dict(xmax=(0.5, -0.5, 0.5))])
+                                 dict(energy_flux=2), dict(precision=3), dict(stages=3),
+                                 dict(xmax=(0.5, -0.5, 0.5))])
 def test_create_rejects_bad_config(bad):
     cfg = dict(nx=16, ny=16, nz=16, **BASE)
     cfg.update(bad)
