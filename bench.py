@@ -28,7 +28,9 @@ of P, between pageable numpy arrays allocated once), and the rate if every step 
 round trip;
 never part of `value`.
 cpu_baseline: the oracle (C++/OpenMP restatement of the reference step) timed on this
-host's cores on 16 steps (about 10 s) of the same 256^3 box.
+host's cores on 16 steps (about 10 s) of the same 256^3 box: one thread per physical core
+the process may use (at most OMP_NUM_THREADS), OMP_PROC_BIND=close, OMP_PLACES=cores, in a
+child process run before the GPU is touched; the line names the CPU model and core counts.
 ensemble / serving: the batched and request-serving measurements (skipped by --no-serve);
 ensemble.rk2 repeats the ensemble with the two-stage step (skipped by --no-rk2 too).
 rk2: the same box advanced by the two-stage Runge-Kutta step at CFL factor 0.3 (the
@@ -93,11 +95,35 @@ def make_box(shape, mode_precision: int, stages: int = 1, cfl: float = 0.05):
     return b
 
 
-def cpu_baseline(shape, steps: int) -> dict:
-    """oracle on this host's cores, same box, `steps` steps (after building its state)"""
+def host_cpu() -> dict:
+    """CPU model, logical CPUs this process may run on, and the physical cores among them"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            topo = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(topo + "physical_package_id") as f1, open(topo + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("cpu", c))
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores)}
+
+
+def _cpu_baseline_worker(shape, steps: int) -> dict:
+    """oracle on this host's cores, same box, `steps` steps (after building its state); runs
+    in the child process cpu_baseline starts with the OpenMP binding in its environment"""
     from oracle import oracle as orc
     nx, ny, nz = shape
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = int(os.environ["OMP_NUM_THREADS"])
     orc.lib().orc_set_threads(threads)
     o = orc.Oracle(nx=nx, ny=ny, nz=nz, bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
                    gam=5.0 / 3.0, cfl=0.05, energy_flux=1, grid_shift=0)
@@ -107,9 +133,28 @@ def cpu_baseline(shape, steps: int) -> dict:
     t0 = time.perf_counter()
     o.run(steps)
     wall = time.perf_counter() - t0
+    return {"value": nx * ny * nz * steps / wall, "wall_s": wall}
+
+
+def cpu_baseline(shape, steps: int) -> dict:
+    """The oracle (C++/OpenMP restatement of the reference step) timed on this host: one
+    thread per physical core this process may use (at most OMP_NUM_THREADS when that is
+    set, the host's CPU share), OMP_PROC_BIND=close and OMP_PLACES=cores, in a child process
+    so that the OpenMP runtime starts with that binding."""
+    import subprocess
+    cpu = host_cpu()
+    threads = cpu["physical_cores"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit() and int(os.environ["OMP_NUM_THREADS"]) > 0:
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", *map(str, shape),
+                        str(steps)], env=env, capture_output=True, text=True, check=True)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    nx, ny, nz = shape
     name = f"{nx}^3" if nx == ny == nz else f"{nx}x{ny}x{nz}"
-    return {"value": nx * ny * nz * steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same {name} box (oracle C++/OpenMP restatement), {wall:.1f} s"}
+    return {"value": res["value"], "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {name} box (oracle C++/OpenMP restatement), {res['wall_s']:.1f} s",
+            "cpu": cpu, "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}}
 
 
 SHEAR_BASE = dict(bc=(1, 1, 2, 2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), gam=5.0 / 3.0, cfl=0.05,
@@ -238,7 +283,13 @@ def main():
                     help="1: the reference's forward-Euler step; 2: two-stage Runge-Kutta")
     ap.add_argument("--cfl", type=float, default=None, help="CFL factor (default 0.05, or 0.3 with --stages 2)")
     ap.add_argument("--no-rk2", action="store_true")
+    ap.add_argument("--cpu-baseline-worker", type=int, nargs=4, metavar=("NX", "NY", "NZ", "STEPS"),
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_worker:
+        nx, ny, nz, steps = args.cpu_baseline_worker
+        print(json.dumps(_cpu_baseline_worker((nx, ny, nz), steps)), flush=True)
+        return
     cfl = args.cfl if args.cfl is not None else (0.3 if args.stages == 2 else 0.05)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole box lives on one MI355X; --gpus {args.gpus} is not supported",
@@ -250,6 +301,13 @@ def main():
     cells = shape[0] * shape[1] * shape[2]
     cube = shape[0] == shape[1] == shape[2]
     shape_name = f"{n}^3" if cube and not args.shape else "x".join(map(str, shape))
+    # the CPU baseline runs first, in a child process started before this process touches
+    # the GPU
+    cpu_line = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {shape_name}")
+        cpu_line = cpu_baseline(shape, args.cpu_steps)
+        log(f"[bench] cpu baseline {cpu_line['value']:.4e} cell-updates/s on {cpu_line['cores']} cores")
     log(f"[bench] building {shape_name} box on the device")
     box = make_box(shape, args.precision, args.stages, cfl)
     log(f"[bench] device bytes {box.device_bytes() / 2**30:.2f} GiB; warmup {args.warmup} steps")
@@ -312,7 +370,7 @@ def main():
         valu = {"kernel": dominant, "lane_instr_per_launch": valu_n, "achieved": rate,
                 "peak": VALU_PEAK_LANE_INSTR_S, "unit": "lane-instr/s", "frac": round(rate / VALU_PEAK_LANE_INSTR_S, 4)}
     out = {
-        "metric": "cell-updates/s (% HBM roofline) at 256^3 FP64",
+        "metric": "cell-updates/s (% HBM roofline) at 256^3 FP64; p99 latency of batched requests",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {0: "f64", 1: "f64 state, f32 reconstruction+HLLD",
@@ -342,11 +400,7 @@ def main():
             out["ensemble"]["rk2"] = ensemble_bench(stages=2)
         log(f"[bench] serving: {args.serve_requests} requests, batched vs one at a time")
         out["serving"] = serving_bench(args.serve_requests)
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {shape_name}")
-        out["cpu_baseline"] = cpu_baseline(shape, args.cpu_steps)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = cpu_line
     print(json.dumps(out), flush=True)
 
 

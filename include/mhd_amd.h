@@ -128,7 +128,13 @@ int mhd_update_primitive(mhd_ctx* ctx);
  * returns the steps applied.  P and U are both valid afterwards.
  * mode: 0 = fused step kernel (default), 1 = staged kernels (one per reference function).
  * With mhd_config.stages = 2 each iteration is the two-stage Runge-Kutta step over that
- * loop body's update (one dt and c_h per iteration, from its starting state). */
+ * loop body's update (one dt and c_h per iteration, from its starting state).
+ * A cell with rho <= 0, or whose CFL bound or fast speed is not finite and positive (a NaN,
+ * or e < 0), stops the loop and sets status bit 0 (mhd_status); the reference checks
+ * nothing and would go on with a NaN dt (src_cpp_kokkos_host/mhd.cpp:1499,1560).  The fused
+ * path stops after the step that produced the cell (its dt/c_h reduction runs on the new
+ * state), the staged path before the step that would read it; *steps_done counts the steps
+ * applied, and the call still returns MHD_OK. */
 int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done);
 
 /* Kelvin-Helmholtz initial state of src_cpp_kokkos_host/main.cpp:66-184 on the device
@@ -142,7 +148,8 @@ int mhd_generate_problem_shear(mhd_ctx* ctx, const mhd_ic_params* ic);
  * out[0]=volume, out[1]=mix, out[2]=sqrt(<vy^2>), out[3]=max|divB| (central differences
  * on cells whose neighbours are interior; the reference has no divB diagnostic). */
 int mhd_diagnostics(mhd_ctx* ctx, double* out4);
-/* status word: bit 0 = a non-finite or non-positive dt / c_h was met */
+/* status word: bit 0 = a cell with rho <= 0 or a non-finite or non-positive CFL bound / fast
+ * speed was met (see mhd_run); it stays set for the life of the context */
 int mhd_status(mhd_ctx* ctx, int* status);
 
 /* kernel timing of the most recent mhd_run: total ms of the step kernels on the
@@ -183,6 +190,8 @@ int mhd_batch_remove_box(mhd_batch* batch, int box_id);
 /* nsteps loop iterations of every live box; returns when they are done */
 int mhd_batch_step(mhd_batch* batch, int nsteps);
 int mhd_batch_last_step_ms(const mhd_batch* batch, double* ms);
+/* status: as mhd_status, for this box since it was added or its state uploaded; a box whose
+ * state holds a bad cell stops on its own (stopped = 1) while the other boxes go on */
 int mhd_batch_box_state(mhd_batch* batch, int box_id, double* time, int* steps_done, int* stopped, int* status);
 /* ids of live boxes that have stopped (n may exceed cap; ids holds the first cap) */
 int mhd_batch_poll(mhd_batch* batch, int* ids, int cap, int* n);

@@ -8,8 +8,13 @@ Each .hip source is compiled to an object with hipcc for gfx950, then linked int
 mhdturbulence_amd/libmhd_amd.so.  The drop-in host layer (mhdturbulence_amd/host/)
 is compiled with g++ against include/mhd_amd.h into libmhd_kh.so plus the example
 driver mhd_kh_main, so that an unmodified src_cpp_kokkos_host-style step loop links
-against the reference's function names.  csrc/mhd_calib.hip becomes the executable
-mhd_calib (HBM stream rates and FP64 issue rate of the GPU it runs on).
+against the reference's function names.
+
+    python -m mhdturbulence_amd.build --calib   # only the measurement tool
+
+builds calib/mhd_calib.hip into the executable mhd_calib (HBM stream rates and FP64 issue
+rate of the GPU it runs on).  It is a measurement tool, not part of the library, and
+build() does not compile it.
 """
 from __future__ import annotations
 
@@ -97,7 +102,7 @@ def build_host(verbose: bool = True) -> str:
 def build_calib(verbose: bool = True) -> str:
     """The calibration executable (stream bandwidth, FP64 issue rate)."""
     tmp = CALIB + ".tmp"
-    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp, os.path.join(CSRC, "mhd_calib.hip")])
+    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp, os.path.join(PKG, "calib", "mhd_calib.hip")])
     os.replace(tmp, CALIB)
     if verbose:
         print(f"built {CALIB}")
@@ -107,7 +112,6 @@ def build_calib(verbose: bool = True) -> str:
 def build(verbose: bool = True) -> None:
     build_hip(verbose)
     build_host(verbose)
-    build_calib(verbose)
 
 
 def main(argv=None):
@@ -118,8 +122,11 @@ def main(argv=None):
                     help="compile switch NAME=VALUE for a build variant (MHD_FAST_DIV, MHD_FAST_SQRT, MHD_XY_DPP, "
                          "MHD_F64_MINWAVES)")
     ap.add_argument("--out", default=LIB, help="library path; a variant elsewhere skips the host layer")
+    ap.add_argument("--calib", action="store_true", help="build only the mhd_calib measurement tool")
     a = ap.parse_args(argv)
-    if a.define or a.out != LIB:
+    if a.calib:
+        build_calib(not a.quiet)
+    elif a.define or a.out != LIB:
         build_hip(not a.quiet, defines=a.define, out=os.path.abspath(a.out))
     else:
         build(not a.quiet)

@@ -73,9 +73,11 @@ uint64_t up(uint64_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
 
 // byte sizes of a box's arrays, in carve order
 constexpr int NREGION = 11;
-void box_layout(const Geom& g, uint64_t* sz) {
+// precision: the batch's mhd_config.precision (2 stores the divergence acc in FP32)
+void box_layout(const Geom& g, int precision, uint64_t* sz) {
   sz[0] = up(mhd::NPRIM * g.vstride * 8);                  // P
-  sz[1] = sz[2] = sz[3] = up(mhd::NCONS * g.vstride * 8);  // U0 U1 acc
+  sz[1] = sz[2] = up(mhd::NCONS * g.vstride * 8);          // U0 U1
+  sz[3] = up(mhd::NCONS * g.vstride * (precision == 2 ? 4 : 8));  // acc
   sz[4] = sz[5] = up((uint64_t)g.itot * 8);                // x1a x1b
   sz[6] = sz[7] = up((uint64_t)g.jtot * 8);                // x2a x2b
   sz[8] = sz[9] = up((uint64_t)g.ktot * 8);                // x3a x3b
@@ -381,7 +383,7 @@ int mhd_batch_box_bytes(const mhd_batch* b, int nx, int ny, int nz, uint64_t* by
   bool fits = false;  // the byte count is reported either way; add_box rejects what does not fit
   (void)configure(const_cast<mhd_batch*>(b), g, &fits);
   uint64_t sz[NREGION];
-  box_layout(g, sz);
+  box_layout(g, b->base.precision, sz);
   uint64_t t = 0;
   for (uint64_t v : sz) t += v;
   *bytes = t;
@@ -428,7 +430,7 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
   FusedArgs a = configure(b, x.g, &fits);
   if (!fits) return fail(MHD_ERR_ARG, "box too large: a variable column spans 4 GiB or more");
   uint64_t sz[NREGION];
-  box_layout(x.g, sz);
+  box_layout(x.g, b->base.precision, sz);
   uint64_t total = 0;
   for (uint64_t v : sz) total += v;
   if (!arena_take(b, total, &x.off)) return fail(MHD_ERR_NOMEM, "arena has no free region that large");
@@ -441,9 +443,10 @@ int mhd_batch_add_box(mhd_batch* b, int nx, int ny, int nz, const mhd_ic_params*
     *dst[q] = reinterpret_cast<double*>(base + o);
     o += sz[q];
   }
-  // staging for host-layout copies spans U1 and acc
+  // staging for host-layout copies spans U1 and acc (at least 15 padded variables, which
+  // hold the 12 unpadded host-layout ones)
   if (reinterpret_cast<char*>(x.acc) != reinterpret_cast<char*>(x.U1) + sz[2] ||
-      2 * sz[2] < (uint64_t)mhd::NPRIM * x.g.itot * x.g.jtot * x.g.ktot * 8)
+      sz[2] + sz[3] < (uint64_t)mhd::NPRIM * x.g.itot * x.g.jtot * x.g.ktot * 8)
     return fail(MHD_ERR_STATE, "box layout: staging area too small");
   // the whole region starts from zero so that the never-read padding is finite
   BHIP(hipMemsetAsync(base, 0, total, b->stream));
@@ -484,8 +487,8 @@ int mhd_batch_upload_state(mhd_batch* b, int id, const double* P, const double* 
   BHIP(hipSetDevice(b->dev));
   BatchBox& x = b->boxes[id];
   const size_t cells = (size_t)x.g.itot * x.g.jtot * x.g.ktot;
-  // U1 and acc are adjacent in the box region (20 variables of padded rows), which holds
-  // the 12 unpadded host-layout variables; neither carries state before the restart
+  // U1 and acc are adjacent in the box region (15 or 20 variables of padded rows), which
+  // hold the 12 unpadded host-layout variables; neither carries state before the restart
   double* stage = x.U1;
   BHIP(hipMemcpyAsync(stage, P, mhd::NPRIM * cells * 8, hipMemcpyHostToDevice, b->stream));
   mhd::launch_from_host_layout(x.g, mhd::NPRIM, stage, x.P, b->stream);
@@ -603,8 +606,8 @@ int mhd_batch_finish_box(mhd_batch* b, int id, double* diag4, double* P_host) {
   }
   if (P_host) {
     const size_t cells = (size_t)x.g.itot * x.g.jtot * x.g.ktot;
-    // U1 + acc (adjacent, 20 padded variables) stage the 12 host-layout variables; the state
-    // is in U0 at this point
+    // U1 + acc (adjacent, 15 or 20 padded variables) stage the 12 host-layout variables; the
+    // state is in U0 at this point
     mhd::launch_to_host_layout(x.g, mhd::NPRIM, x.P, x.U1, b->stream);
     BHIP(hipMemcpyAsync(P_host, x.U1, mhd::NPRIM * cells * 8, hipMemcpyDeviceToHost, b->stream));
   }

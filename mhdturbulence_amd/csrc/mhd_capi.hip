@@ -117,7 +117,8 @@ int ensure_fused(mhd_ctx* c) {
     return fail(MHD_ERR_UNSUPPORTED, "fused step: a variable column of this box spans 4 GiB or more "
                                      "(more than 2^29 cells); use the staged step (mode 1)");
   CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
-  CHKRET(dalloc(c, &c->acc, mhd::NCONS * c->g.vstride));
+  // the divergence in FP32 (precision 2) takes half the bytes
+  CHKRET(dalloc(c, &c->acc, a.acc32 ? (mhd::NCONS * c->g.vstride + 1) / 2 : mhd::NCONS * c->g.vstride));
   a.x1a = c->grid[0];
   a.x2a = c->grid[2];
   a.x3a = c->grid[4];
@@ -547,7 +548,8 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
   HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
   int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
   HIPCHK(hipMemcpyAsync(&c->sc->iter, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, &c->dt_log_cap, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const int cap_halt[2] = {c->dt_log_cap, 0};  // dt_log_cap, halt
+  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, cap_halt, sizeof cap_halt, hipMemcpyHostToDevice, c->stream));
   mhd::launch_reset_scalars(c->sc, 1, c->stream);
   if (mode == 0) {
     // the first iteration's dt and c_h from P (ControlTimestep / EvaluateCh on the state as

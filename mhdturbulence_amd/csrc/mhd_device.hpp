@@ -45,13 +45,20 @@ struct StepScalars {
   int iter;          // loop iterations issued since the last reset
   int steps_done;    // steps actually applied
   int stopped;
-  int status;        // bit 0: a non-finite or non-positive dt / c_h was met
+  int status;        // bit 0: a cell with rho <= 0 or a non-finite dt / c_h bound was met (sticky)
   int dt_log_cap;
-  int pad;
+  int halt;          // the step just applied left such a cell: the loop stops after it
 };
 
 constexpr unsigned long long DT_INIT_BITS = 0x7FF0000000000000ULL;  // +inf
 constexpr unsigned long long CH_INIT_BITS = 0ULL;                   // +0.0
+
+// A cell the step cannot continue from: rho <= 0, or a CFL bound / fast speed that is not
+// finite and positive (a NaN anywhere in the cell, or e < 0 whose sound speed is a NaN).
+// Every comparison is false for a NaN, so each test is written to fail on one.
+__device__ __forceinline__ bool bad_cell(double rho, double cdt, double cch) {
+  return !(rho > 0.0) || !(cdt > 0.0) || !(cch <= 1.7976931348623157e308);
+}
 
 // ---- launch wrappers (mhd_kernels.hip) ----
 void launch_from_host_layout(const Geom& g, int nv, const double* src, double* dst, hipStream_t s);
@@ -67,7 +74,9 @@ void launch_problem_shear(const Geom& g, double gam, const double* ic /* rho0,p0
 // sc->host_acc := (+inf, 0); or, ring mode, all three slots
 void launch_reset_scalars(StepScalars* sc, int ring, hipStream_t s);
 // min-dt / max-c_h reduction over the interior cells of P.  ring=0: into sc->host_acc;
-// ring=1: into sc->red[sc->iter % 3].
+// ring=1: into sc->red[sc->iter % 3], the bounds of the step about to be taken; a bad_cell
+// then also sets sc->stopped, so that step and every later one of the loop are skipped.
+// Either way a bad_cell sets status bit 0.
 void launch_reduce_prim(const Geom& g, const double* P, const double* x1a, const double* x2a, const double* x3a,
                         StepScalars* sc, int ring, hipStream_t s);
 // ghost fill of an nv-variable array (P: nv = 12, U: nv = 10)

@@ -731,6 +731,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fzprev[m] = 0.0;
   double dtl = __builtin_inf(), chl = 0.0;
+  bool bad = false;  // an updated cell of this lane is a bad_cell
 
   // Pass k: cell k-1 is updated.  The raw state of plane k+2 and the old state and
   // divergence of cell k-1 are loaded at the start; plane k+2 becomes primitives after the
@@ -807,8 +808,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
       if constexpr (reduce) {
         double q[NPRIM];
         cons_to_prim(u, a.gam, q);
-        dtl = fmin(dtl, cell_dt(q, dxa, dya, dza));
-        chl = fmax(chl, cell_ch(q));
+        const double cdt = cell_dt(q, dxa, dya, dza), cch = cell_ch(q);
+        bad |= bad_cell(q[nden], cdt, cch);
+        dtl = fmin(dtl, cdt);
+        chl = fmax(chl, cch);
       }
     }
     const int t0 = sm;
@@ -819,20 +822,22 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     for (int m = 0; m < NCONS; ++m) Fzprev[m] = Fz[m];
   }
   if constexpr (!reduce) return;
-  const bool bad = !(dtl > 0.0) || !(chl == chl) || isinf(chl);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     dtl = fmin(dtl, __shfl_xor(dtl, o, 64));
     chl = fmax(chl, __shfl_xor(chl, o, 64));
   }
-  // every wave updates some cells, so an infinite wave minimum means no finite bound (NaN state)
-  const unsigned long long anybad = __ballot(bad) | (isinf(dtl) ? 1ull : 0ull);
+  const unsigned long long anybad = __ballot(bad);
   if (lane == 0) {
     unsigned long long* nx = sc->red[(iter + 1) % 3];
     if (!(chl == chl)) chl = __builtin_inf();
     atomicMin(&nx[0], d2bits(dtl));
     atomicMax(&nx[1], d2bits(chl));
-    if (anybad) atomicOr(&sc->status, 1);
+    // the step-end launch stops the loop after this step (the new state holds the cell)
+    if (anybad) {
+      atomicOr(&sc->status, 1);
+      atomicExch(&sc->halt, 1);
+    }
   }
 }
 
@@ -898,6 +903,7 @@ __global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int*
   nxt[1] = CH_INIT_BITS;
   if (dt_last) dt_last[b] = dt;
   if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
+  if (sc->halt) sc->stopped = 1;  // the state this step left holds a bad cell
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

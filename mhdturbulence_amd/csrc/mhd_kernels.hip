@@ -164,13 +164,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Block-level (min dt, max ch) -> one atomic pair per block.
-__device__ __forceinline__ void block_commit(double dtl, double chl, unsigned long long* acc, int* status) {
+// Block-level (min dt, max ch) -> one atomic pair per block.  bad: this thread met a
+// bad_cell; stop: also stop the device loop (the bounds are those of the step about to run).
+__device__ __forceinline__ void block_commit(double dtl, double chl, bool bad, unsigned long long* acc,
+                                             StepScalars* sc, bool stop) {
   __shared__ double s_dt[BLK / 64], s_ch[BLK / 64];
   __shared__ int s_bad;
   if (threadIdx.x == 0) s_bad = 0;
   __syncthreads();
-  const bool bad = !(dtl > 0.0) || !(chl == chl) || isinf(chl);
   if (bad) s_bad = 1;
   dtl = wave_min(dtl);
   chl = wave_max(chl);
@@ -183,7 +184,10 @@ __device__ __forceinline__ void block_commit(double dtl, double chl, unsigned lo
     if (!(c == c)) c = __builtin_inf();
     atomicMin(&acc[0], double_to_bits(d));
     atomicMax(&acc[1], double_to_bits(c));
-    if (s_bad && status) atomicOr(status, 1);
+    if (s_bad) {
+      atomicOr(&sc->status, 1);
+      if (stop) atomicExch(&sc->stopped, 1);
+    }
   }
 }
 
@@ -192,6 +196,7 @@ __global__ __launch_bounds__(BLK) void k_reduce_prim(Geom g, const double* __res
                                                     const double* __restrict__ x3a, StepScalars* sc, int ring) {
   const long long ncell = (long long)g.nx * g.ny * g.nz;
   double dtl = __builtin_inf(), chl = 0.0;
+  bool bad = false;
   for (long long t = (long long)blockIdx.x * BLK + threadIdx.x; t < ncell; t += (long long)gridDim.x * BLK) {
     const int i = NGH + (int)(t % g.nx);
     const long long r = t / g.nx;
@@ -201,11 +206,14 @@ __global__ __launch_bounds__(BLK) void k_reduce_prim(Geom g, const double* __res
     double q[NPRIM];
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) q[n] = P[n * g.vstride + c];
-    dtl = fmin(dtl, cell_dt(q, x1a[i + 1] - x1a[i], x2a[j + 1] - x2a[j], x3a[k + 1] - x3a[k]));
-    chl = fmax(chl, cell_ch(q));
+    const double cdt = cell_dt(q, x1a[i + 1] - x1a[i], x2a[j + 1] - x2a[j], x3a[k + 1] - x3a[k]);
+    const double cch = cell_ch(q);
+    bad |= bad_cell(q[nden], cdt, cch);
+    dtl = fmin(dtl, cdt);
+    chl = fmax(chl, cch);
   }
   unsigned long long* acc = ring ? sc->red[sc->iter % 3] : sc->host_acc;
-  block_commit(dtl, chl, acc, &sc->status);
+  block_commit(dtl, chl, bad, acc, sc, ring != 0);
 }
 
 __global__ void k_reset_scalars(StepScalars* sc, int ring) {
@@ -384,6 +392,9 @@ __global__ void k_step_end(StepScalars* sc, double cfl, double* dt_log, int ahea
   nxt[0] = DT_INIT_BITS;
   nxt[1] = CH_INIT_BITS;
   if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
+  // the fused loop reduces the next step's bounds in the step it applies: a bad cell in the
+  // new state stops the loop after this step (the staged loop stops before the step instead)
+  if (sc->halt) sc->stopped = 1;
 }
 
 // ---------------------------------------------------------------- diagnostics

@@ -44,6 +44,10 @@ class Result:
     steps: int
     sim_time: float
     diagnostics: list = field(default_factory=list)
+    # mhd_batch_box_state status: 0 = clean; bit 0 = the box met a cell with rho <= 0 or a
+    # non-finite CFL bound and stopped there, before its t_end
+    status: int = 0
+    prim: np.ndarray | None = None  # final primitive state (reference layout), if kept
 
     @property
     def latency(self) -> float:
@@ -116,7 +120,9 @@ class Server:
     def footprint(self, n: int) -> int:
         return self.batch.box_bytes(n, n, n)
 
-    def serve(self, requests: list[Request], clock=time.perf_counter, log=None) -> dict:
+    def serve(self, requests: list[Request], clock=time.perf_counter, log=None, keep_prim: bool = False) -> dict:
+        """keep_prim: each Result also carries its final primitive state (downloaded as the box
+        retires; for small boxes, as in tests)"""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = clock()
         nxt = 0
@@ -148,10 +154,11 @@ class Server:
             for bid in self.batch.poll():
                 r, adm = live.pop(bid)
                 st = self.batch.box_state(bid)
-                diag, _ = self.batch.finish_box(bid)
+                diag, prim = self.batch.finish_box(bid, want_prim=keep_prim)
                 self.batch.remove_box(bid)
                 cell_updates += st["steps"] * r.n ** 3
-                res = Result(r.rid, r.n, r.arrival, adm, clock() - t0, st["steps"], st["time"], diag.tolist())
+                res = Result(r.rid, r.n, r.arrival, adm, clock() - t0, st["steps"], st["time"], diag.tolist(),
+                             st["status"], prim)
                 results.append(res)
                 if log:
                     log(f"done rid={r.rid} n={r.n} steps={st['steps']} latency={res.latency:.3f}s")
@@ -166,5 +173,6 @@ class Server:
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()),
+            "failed": sum(1 for r in results if r.status),
             "results": results,
         }

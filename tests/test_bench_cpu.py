@@ -21,3 +21,16 @@ def test_documented_flags_are_accepted():
     for flag in ("--gpus", "--steps", "--warmup", "--n", "--shape", "--mode", "--precision", "--no-cpu", "--no-serve",
                  "--profile-steps", "--cpu-steps", "--serve-requests"):
         assert flag in r.stdout, flag
+
+
+def test_cpu_baseline_binds_threads_and_names_the_cpu():
+    """the baseline leg runs the oracle in a child process with one thread per physical core
+    (at most OMP_NUM_THREADS), close binding on cores, and reports the CPU it ran on"""
+    sys.path.insert(0, ROOT)
+    import bench
+    line = bench.cpu_baseline((16, 12, 8), 2)
+    cpu = bench.host_cpu()
+    assert line["kind"] == "port" and line["unit"] == "cell-updates/s" and line["value"] > 0
+    assert 1 <= line["cores"] <= cpu["physical_cores"] <= cpu["logical_cpus"]
+    assert line["omp"] == {"OMP_NUM_THREADS": line["cores"], "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
+    assert line["cpu"]["model"] == cpu["model"]

@@ -353,7 +353,28 @@ def test_diagnostics_match(orc):
     b.run(20, mode=0)
     do, dg = o.diagnostics(), b.diagnostics()
     np.testing.assert_allclose(dg[:3], do[:3], rtol=1e-12)
-    assert dg[3] == pytest.approx(do[3], rel=1e-6, abs=1e-12)
+    assert dg[3] == pytest.approx(do[3], rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["linear", "quadratic", "quadratic_low", "sine", "solenoidal"])
+def test_divb_closed_form_on_device(orc, kind):
+    """mhd_diagnostics of fields whose discrete divergence is known in closed form
+    (tests/divb_fields.py), on non-uniform anisotropic grids uploaded with mhd_set_grid"""
+    import divb_fields as dvf
+    from test_oracle import _divb_box
+    n = (14, 11, 9)
+    o = _divb_box(orc, kind, n)
+    b = Box(Config(nx=n[0], ny=n[1], nz=n[2], bc=(1, 1, 1, 1, 1, 1), xmin=(0.3, 0.2, 0.1), xmax=(1.3, 1.0, 0.7),
+                   grid_shift=0))
+    b.set_grid(o.grid(1), o.grid(1, True), o.grid(2), o.grid(2, True), o.grid(3), o.grid(3, True))
+    b.upload_prim(o.P)
+    dg, do = b.diagnostics(), o.diagnostics()
+    want = dvf.expected_max_divb(kind, o.grid(1, True), o.grid(2, True), o.grid(3, True), n)
+    if kind == "solenoidal":
+        assert dg[3] == 0.0
+    else:
+        assert dg[3] == pytest.approx(want, rel=1e-13), (kind, dg[3], want)
+    np.testing.assert_allclose(dg[:3], do[:3], rtol=1e-13)
 
 
 @pytest.mark.parametrize("precision", [1, 2])

@@ -239,6 +239,47 @@ def test_initial_state_matches_vissimple_snapshot(orc):
             assert got == pytest.approx(ref, rel=5.1e-3, abs=1e-300), (x, y, got, ref)
 
 
+def _divb_box(orc, kind, n=(14, 11, 9)):
+    """an oracle box holding divb_fields' field `kind`: non-uniform grids (a different one per
+    axis) for the polynomial and solenoidal fields, the generated uniform grid for the sine"""
+    import divb_fields as dvf
+    o = orc.Oracle(nx=n[0], ny=n[1], nz=n[2], bc=(1, 1, 1, 1, 1, 1), xmin=(0.3, 0.2, 0.1), xmax=(1.3, 1.0, 0.7),
+                   grid_shift=0)
+    o.generate_grid()
+    if kind != "sine":
+        for ax, (lo, ln, seed) in enumerate(((0.3, 1.0, 1), (0.2, 0.8, 2), (0.1, 0.6, 3))):
+            f = dvf.nonuniform_faces(n[ax], lo, ln, seed)
+            o.grid(ax + 1)[:] = f
+            o.grid(ax + 1, True)[:] = dvf.centres(f)
+    dvf.fill(o.P, o.grid(1, True), o.grid(2, True), o.grid(3, True), kind)
+    return o
+
+
+@pytest.mark.parametrize("kind", ["linear", "quadratic", "quadratic_low", "sine", "solenoidal"])
+def test_divb_closed_form(orc, kind):
+    """max|divB| of the oracle's diagnostics against the closed-form central difference of
+    fields whose discrete divergence is known (tests/divb_fields.py), on non-uniform and
+    anisotropic grids; the volume is the domain's and the sums are those of the cells"""
+    import divb_fields as dvf
+    n = (14, 11, 9)
+    o = _divb_box(orc, kind, n)
+    d = o.diagnostics()
+    want = dvf.expected_max_divb(kind, o.grid(1, True), o.grid(2, True), o.grid(3, True), n)
+    if kind == "solenoidal":
+        assert d[3] == 0.0
+    else:
+        assert d[3] == pytest.approx(want, rel=1e-13), (kind, d[3], want)
+    assert d[0] == pytest.approx(1.0 * 0.8 * 0.6, rel=1e-13)
+    # the sums are over interior cells, weighted by cell volume
+    xa, ya, za = o.grid(1), o.grid(2), o.grid(3)
+    dv = (np.diff(za)[2:2 + n[2], None, None] * np.diff(ya)[None, 2:2 + n[1], None] *
+          np.diff(xa)[None, None, 2:2 + n[0]])
+    Pi = o.interior(o.P)
+    X, vy = Pi[11], Pi[2]
+    assert d[1] == pytest.approx((X * (1 - X) * dv).sum() / dv.sum(), rel=1e-13)
+    assert d[2] == pytest.approx(np.sqrt((vy * vy * dv).sum() / dv.sum()), rel=1e-13)
+
+
 def test_dt_and_ch_closed_form_uniform(orc):
     o = _box(orc, n=(8, 8, 8), xmin=(0, 0, 0), xmax=(1, 2, 4))
     P = o.P
@@ -366,48 +407,64 @@ def _rk2_step(o, dt, ch):
         o.update_primitive()
 
 
-def _linear_wave_error(orc, n, wave, rk2=False):
-    """L1 error of a small-amplitude wave along x after a quarter crossing.
+def _linear_wave_error(orc, n, wave, rk2=False, axis=0):
+    """L1 error of a small-amplitude wave along `axis` (0 x, 1 y, 2 z) after a quarter crossing.
 
     The reference advances with one forward-Euler stage per step (first order in time), so
     the CFL number is scaled with 1/n to expose the second-order spatial discretisation.
-    rk2: the two-stage Runge-Kutta step at a fixed CFL number of 0.3, second order in time."""
-    o = orc.Oracle(nx=n, ny=4, nz=4, bc=(1, 1, 1, 1, 1, 1), xmin=(0, 0, 0), xmax=(1, 4.0 / n, 4.0 / n),
+    rk2: the two-stage Runge-Kutta step at a fixed CFL number of 0.3, second order in time.
+    The wave axis has n cells on [0, 1]; each transverse axis has 4 cells, 1.5/n and 0.75/n
+    wide (the box is anisotropic, so a y or z spacing taken from the wrong axis changes the
+    wave's speed and the error stops converging)."""
+    widths = [1.0 / n] * 3
+    widths[(axis + 1) % 3] = 1.5 / n
+    widths[(axis + 2) % 3] = 0.75 / n
+    ncell = [4, 4, 4]
+    ncell[axis] = n
+    o = orc.Oracle(nx=ncell[0], ny=ncell[1], nz=ncell[2], bc=(1, 1, 1, 1, 1, 1), xmin=(0, 0, 0),
+                   xmax=tuple(w * c for w, c in zip(widths, ncell)),
                    gam=5.0 / 3.0, cfl=0.3 if rk2 else 0.1 * 32.0 / n, energy_flux=1, grid_shift=0,
                    stages=2 if rk2 else 1)
     o.generate_grid()
     gam = 5.0 / 3.0
     # p0 = 0.6/gam keeps c_s != c_A: at c_s = c_A the fast-speed discriminant of the reference
     # (cts^2 - 4 cs^2 Bn^2/rho, no guard) rounds below zero and its sqrt is NaN
-    rho0, p0, bx, by = 1.0, 0.6 / gam, 1.0, 0.0
-    x = o.grid(1, True)
+    rho0, p0, bn = 1.0, 0.6 / gam, 1.0
+    x = o.grid(axis + 1, True)
     amp = 1e-6
     P = o.P
     k = 2 * np.pi
+    zero = 0 * x
     if wave == "sound":
         cs = np.sqrt(gam * p0 / rho0)
-        bx = 0.0
+        bn = 0.0
         drho = amp * np.sin(k * x)
         rho = rho0 + drho
-        vx = cs * drho / rho0
+        vn = cs * drho / rho0
         p = p0 + cs * cs * drho
-        speed, vy, byv = cs, 0 * x, 0 * x
-    else:  # Alfven wave along B = (1, 0, 0): vy = -dBy / sqrt(rho)
-        speed = bx / np.sqrt(rho0)
-        rho, vx, p = rho0 + 0 * x, 0 * x, p0 + 0 * x
-        byv = amp * np.sin(k * x)
-        vy = -byv / np.sqrt(rho0)
-    for kk in range(o.ktot - 1):
-        for jj in range(o.jtot - 1):
-            P[kk, jj, 0, :] = rho
-            P[kk, jj, 1, :] = vx
-            P[kk, jj, 2, :] = vy
-            P[kk, jj, 5, :] = p
-            P[kk, jj, 4, :] = p / ((gam - 1) * rho)
-            P[kk, jj, 6, :] = np.sqrt(gam * p / rho)
-            P[kk, jj, 7, :] = bx
-            P[kk, jj, 8, :] = byv
-            P[kk, jj, 11, :] = 0.5
+        speed, vt, bt = cs, zero, zero
+    else:  # Alfven wave along B = bn e_axis: v_t = -dB_t / sqrt(rho)
+        speed = bn / np.sqrt(rho0)
+        rho, vn, p = rho0 + zero, zero, p0 + zero
+        bt = amp * np.sin(k * x)
+        vt = -bt / np.sqrt(rho0)
+    # broadcast a profile along the wave axis over P[k, j, n, i]
+    shape = [(1, 1, -1), (1, -1, 1), (-1, 1, 1)][axis]
+
+    def put(nv, prof):
+        P[:, :, nv, :] = np.broadcast_to(np.reshape(prof, shape), (o.ktot, o.jtot, o.itot))
+
+    P[...] = 0.0
+    t_ax = (axis + 1) % 3
+    put(0, rho)
+    put(1 + axis, vn)
+    put(1 + t_ax, vt)
+    put(5, p)
+    put(4, p / ((gam - 1) * rho))
+    put(6, np.sqrt(gam * p / rho))
+    put(7 + axis, bn + zero)
+    put(7 + t_ax, bt)
+    put(11, 0.5 + zero)
     o.prim_to_cons_all()
     t_end = 0.25 / speed
     t = 0.0
@@ -427,18 +484,30 @@ def _linear_wave_error(orc, n, wave, rk2=False):
             o.update_primitive()
         t += dt
     xs = x[2:2 + n] - speed * t_end
+    line = [lambda v: o.P[2, 2, v, 2:2 + n], lambda v: o.P[2, 2:2 + n, v, 2], lambda v: o.P[2:2 + n, 2, v, 2]][axis]
     if wave == "sound":
         exact = rho0 + amp * np.sin(k * xs)
-        got = o.P[2, 2, 0, 2:2 + n]
+        got = line(0)
     else:
         exact = amp * np.sin(k * xs)
-        got = o.P[2, 2, 8, 2:2 + n]
+        got = line(7 + t_ax)
     return np.mean(np.abs(got - exact)) / amp
 
 
 @pytest.mark.parametrize("wave", ["sound", "alfven"])
 def test_second_order_convergence(orc, wave):
     e = [_linear_wave_error(orc, n, wave) for n in (32, 64, 128)]
+    orders = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    assert np.all(orders > 1.6), (e, orders)
+
+
+@pytest.mark.parametrize("axis", [1, 2])
+@pytest.mark.parametrize("wave", ["sound", "alfven"])
+def test_second_order_convergence_y_and_z(orc, wave, axis):
+    """the same waves along y and along z on anisotropic boxes: pins the y and z flux,
+    reconstruction and update paths (their spacings and component rotations), which exact
+    conservation cannot see"""
+    e = [_linear_wave_error(orc, n, wave, axis=axis) for n in (32, 64, 128)]
     orders = np.log2(np.array(e[:-1]) / np.array(e[1:]))
     assert np.all(orders > 1.6), (e, orders)
 
