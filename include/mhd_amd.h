@@ -137,6 +137,22 @@ int mhd_update_primitive(mhd_ctx* ctx);
  * applied, and the call still returns MHD_OK. */
 int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done);
 
+/* Allocation-free variant of mhd_run in mode 0 (the fused step).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating library routine: the fused loop's scratch (its second copy of the
+ * conserved state, the flux divergence and a dt log of 4096 entries per chunk of steps)
+ * is a static device array of MHD_NOALLOC_SCRATCH_BYTES that each GPU the library runs on
+ * holds until the process ends.  The caller's context holds the inputs and outputs (P, U,
+ * grid, device scalars; created beforehand with mhd_create).  A box whose scratch exceeds
+ * the array (about 80^3 cells in FP64) is refused with MHD_ERR_ARG and a message.
+ * Arguments, results and stop rules are those of mhd_run with mode 0 (dt_log, if not NULL,
+ * has room for nsteps entries).  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other host threads, other contexts' streams) race: issue them
+ * one at a time.  The call returns after its steps are done. */
+#define MHD_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int mhd_run_noalloc(mhd_ctx* ctx, int nsteps, double tmax, double* time, double* dt_log, int* steps_done);
+
 /* Kelvin-Helmholtz initial state of src_cpp_kokkos_host/main.cpp:66-184 on the device
  * (vy amplitude vy_amp: 1e-3 there, 1e-2 in src_cpp_omp_device; harmonic vx kick rrv). */
 int mhd_generate_problem_kh(mhd_ctx* ctx, double vy_amp, double rrv);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -107,32 +108,52 @@ int ensure_flux(mhd_ctx* c) {
   return MHD_OK;
 }
 
-int ensure_fused(mhd_ctx* c) {
-  if (c->fa_ready) return MHD_OK;
-  mhd::FusedArgs& a = c->fa;
+// the fused kernels' arguments for this box over the given second state copy and divergence
+// buffer; allocates nothing.  The tiling is the one mhd_set_tiling chose, if any.
+int fused_args(const mhd_ctx* c, double* U2, double* acc, mhd::FusedArgs& a) {
+  a = mhd::FusedArgs{};
   a.g = c->g;
   a.mixed = c->cfg.precision != 0;
   a.acc32 = c->cfg.precision == 2;
   if (!mhd::fused_configure(a))
     return fail(MHD_ERR_UNSUPPORTED, "fused step: a variable column of this box spans 4 GiB or more "
                                      "(more than 2^29 cells); use the staged step (mode 1)");
-  CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
-  // the divergence in FP32 (precision 2) takes half the bytes
-  CHKRET(dalloc(c, &c->acc, a.acc32 ? (mhd::NCONS * c->g.vstride + 1) / 2 : mhd::NCONS * c->g.vstride));
+  if (c->fa_ready) {
+    a.sw = c->fa.sw;
+    a.ns = c->fa.ns;
+    a.ly = c->fa.ly;
+    a.lz = c->fa.lz;
+  }
   a.x1a = c->grid[0];
   a.x2a = c->grid[2];
   a.x3a = c->grid[4];
   a.U0 = c->U;
-  a.U1 = c->U2;
-  a.acc = c->acc;
+  a.U1 = U2;
+  a.acc = acc;
   a.sc = c->sc;
   a.gam = c->cfg.gam;
   a.cfl = c->cfg.cfl;
   a.alphabp = c->cfg.alphabp;
   for (int q = 0; q < 6; ++q) a.bc[q] = c->cfg.bc[q];
   a.eflux = c->cfg.energy_flux;
-  a.mixed = c->cfg.precision != 0;
-  a.acc32 = c->cfg.precision == 2;
+  return MHD_OK;
+}
+
+// doubles of the divergence buffer (FP32 at precision 2: half the bytes)
+size_t acc_doubles(const mhd_ctx* c) {
+  const size_t n = mhd::NCONS * c->g.vstride;
+  return c->cfg.precision == 2 ? (n + 1) / 2 : n;
+}
+
+int ensure_fused(mhd_ctx* c) {
+  if (c->fa_ready) return MHD_OK;
+  mhd::FusedArgs a;
+  CHKRET(fused_args(c, nullptr, nullptr, a));
+  CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
+  CHKRET(dalloc(c, &c->acc, acc_doubles(c)));
+  a.U1 = c->U2;
+  a.acc = c->acc;
+  c->fa = a;
   c->fa_ready = true;
   return MHD_OK;
 }
@@ -464,7 +485,8 @@ int mhd_update_primitive(mhd_ctx* c) {
 // marks (may be null) receives an event after each launch, for per-kernel timing.
 // stages = 2: the two-stage Runge-Kutta step, U* = E(U^n), U^{n+1} = (U^n + E(U*)) / 2 with
 // E the reference's forward-Euler stage and the dt and c_h of U^n in both stages.
-static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
+// fa (mode 0) and dt_log: the fused kernels' arguments and the device dt log of the run.
+static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks, const mhd::FusedArgs& fa, double* dt_log) {
   const Geom& g = c->g;
   const bool rk2 = two_stage(c);
   int q = 0;
@@ -473,7 +495,7 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
     ++q;
   };
   if (mode == 0) {
-    mhd::FusedArgs a = c->fa;
+    mhd::FusedArgs a = fa;
     a.stage = rk2 ? 1 : 0;
     for (int st = 0; st < (rk2 ? 2 : 1); ++st) {
       if (st == 1) a.stage = 2;
@@ -482,7 +504,7 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
       mhd::launch_fused_z(a, c->stream);
       mark();
     }
-    mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 1, c->stream);
+    mhd::launch_step_end(c->sc, c->cfg.cfl, dt_log, 1, c->stream);
     mark();
     HIPCHK(hipGetLastError());
     return MHD_OK;
@@ -503,7 +525,7 @@ static int issue_step(mhd_ctx* c, int mode, hipEvent_t* marks) {
     mhd::launch_update_prim(g, c->cfg.gam, c->U, c->P, c->sc, c->stream);
     mark();
   }
-  mhd::launch_step_end(c->sc, c->cfg.cfl, c->dt_log, 0, c->stream);
+  mhd::launch_step_end(c->sc, c->cfg.cfl, dt_log, 0, c->stream);
   mark();
   HIPCHK(hipGetLastError());
   return MHD_OK;
@@ -529,6 +551,26 @@ static int launch_kind(const mhd_ctx* c, int mode, int q) {
   return q == 0 ? 0 : q == 13 ? 7 : 1 + (q - 1) % 6;
 }
 
+// the loop scalars of a run starting at `time` with a device dt log of dt_cap entries, and
+// (mode 0) the first iteration's dt / c_h from P and the ghost cells of U; allocates nothing
+static int start_loop(mhd_ctx* c, int mode, double tmax, double time, int dt_cap) {
+  double tt[2] = {time, tmax};
+  HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
+  int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
+  HIPCHK(hipMemcpyAsync(&c->sc->iter, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
+  const int cap_halt[2] = {dt_cap, 0};  // dt_log_cap, halt
+  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, cap_halt, sizeof cap_halt, hipMemcpyHostToDevice, c->stream));
+  mhd::launch_reset_scalars(c->sc, 1, c->stream);
+  if (mode == 0) {
+    // the first iteration's dt and c_h from P (ControlTimestep / EvaluateCh on the state as
+    // given), and the ghost cells of U, which the fused kernels read
+    mhd::launch_reduce_prim(c->g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
+    mhd::launch_boundary(c->g, c->cfg.bc, c->U, mhd::NCONS, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return MHD_OK;
+}
+
 // loop state and buffers for nsteps iterations starting at *time
 static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
   CHKRET(need_grid(c));
@@ -544,41 +586,34 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
   if (mode == 0) CHKRET(ensure_fused(c));
   else CHKRET(ensure_flux(c));
   if (mode == 1 && two_stage(c)) CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
-  double tt[2] = {time, tmax};
-  HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
-  int ints[3] = {0, 0, 0};  // iter, steps_done, stopped
-  HIPCHK(hipMemcpyAsync(&c->sc->iter, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
-  const int cap_halt[2] = {c->dt_log_cap, 0};  // dt_log_cap, halt
-  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, cap_halt, sizeof cap_halt, hipMemcpyHostToDevice, c->stream));
-  mhd::launch_reset_scalars(c->sc, 1, c->stream);
-  if (mode == 0) {
-    // the first iteration's dt and c_h from P (ControlTimestep / EvaluateCh on the state as
-    // given), and the ghost cells of U, which the fused kernels read
-    mhd::launch_reduce_prim(c->g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
-    mhd::launch_boundary(c->g, c->cfg.bc, c->U, mhd::NCONS, c->stream);
-  }
-  HIPCHK(hipGetLastError());
-  return MHD_OK;
+  return start_loop(c, mode, tmax, time, c->dt_log_cap);
 }
 
-static int finish_run(mhd_ctx* c, int mode, int nsteps, double* time, double* dt_log, int* steps_done) {
+// U2: the fused loop's second state copy; dev_log: the run's device dt log.  Waits for the
+// stream; allocates nothing.
+static int finish_loop(mhd_ctx* c, int mode, const double* U2, const double* dev_log, int nsteps, double* time,
+                       double* dt_log, int* steps_done) {
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   if (mode == 0) {
     // one-stage iterations leave the state in U2 after an odd number of them; P is
     // recovered from it (two-stage iterations end in U)
     if (!two_stage(c) && (h.iter & 1))
-      HIPCHK(hipMemcpyAsync(c->U, c->U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
+      HIPCHK(hipMemcpyAsync(c->U, U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
                             c->stream));
     mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (dt_log && h.steps_done > 0)
-    HIPCHK(hipMemcpy(dt_log, c->dt_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dt_log, dev_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
   if (time) *time = h.time;
   if (steps_done) *steps_done = h.steps_done;
   return MHD_OK;
+}
+
+static int finish_run(mhd_ctx* c, int mode, int nsteps, double* time, double* dt_log, int* steps_done) {
+  return finish_loop(c, mode, c->U2, c->dt_log, nsteps, time, dt_log, steps_done);
 }
 
 int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done) {
@@ -588,7 +623,7 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
   if (!c->graph || c->graph_mode != mode || c->graph_log != c->dt_log) {
     drop_graph(c);
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = issue_step(c, mode, nullptr);
+    int rc = issue_step(c, mode, nullptr, c->fa, c->dt_log);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != MHD_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -613,6 +648,83 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
   return MHD_OK;
 }
 
+// ---- the allocation-free run (mhd_run_noalloc) ----
+// Its scratch (the fused loop's second state copy, the flux divergence and a dt log chunk)
+// lives in this array, one copy per GPU the library runs on.  Host code never names the
+// array itself (on the host its name is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns, taken once per device with that device current.
+static __device__ __align__(256) char scratch[MHD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_LOG = 4096;  // dt log entries per chunk of steps
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// this device's copy of `scratch` (the device is current)
+int scratch_base(int dev, char** base) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(MHD_ERR_UNSUPPORTED, "device ordinal above the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MHD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *base = static_cast<char*>(g_scratch_ptr[dev]);
+  return MHD_OK;
+}
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+int mhd_run_noalloc(mhd_ctx* c, int nsteps, double tmax, double* time, double* dt_log, int* steps_done) {
+  if (!time) return fail(MHD_ERR_ARG, "a time pointer is required");
+  CHKRET(need_grid(c));
+  if (nsteps < 0) return fail(MHD_ERR_ARG, "nsteps must be >= 0");
+  const size_t u2_bytes = up256(mhd::NCONS * c->g.vstride * sizeof(double));
+  const size_t acc_bytes = up256(acc_doubles(c) * sizeof(double));
+  const size_t need = u2_bytes + acc_bytes + NOALLOC_LOG * sizeof(double);
+  if (need > (size_t)MHD_NOALLOC_SCRATCH_BYTES)
+    return fail(MHD_ERR_ARG, "mhd_run_noalloc: this box needs " + std::to_string(need) + " bytes of scratch, above " +
+                                 std::to_string((size_t)MHD_NOALLOC_SCRATCH_BYTES) +
+                                 " (MHD_NOALLOC_SCRATCH_BYTES); use mhd_run");
+  char* base = nullptr;
+  CHKRET(scratch_base(c->dev, &base));
+  double* U2 = reinterpret_cast<double*>(base);
+  double* acc = reinterpret_cast<double*>(base + u2_bytes);
+  double* dev_log = reinterpret_cast<double*>(base + u2_bytes + acc_bytes);
+  mhd::FusedArgs a;
+  CHKRET(fused_args(c, U2, acc, a));
+  // the array keeps the previous call's contents: the second state copy and the divergence
+  // start from zero, as a freshly allocated pair does (padding and the never-used last row
+  // and plane are read as whole lines and must be finite)
+  mhd::launch_zero(U2, u2_bytes / sizeof(double), c->stream);
+  mhd::launch_zero(acc, acc_bytes / sizeof(double), c->stream);
+  HIPCHK(hipGetLastError());
+  double t = *time;
+  int done = 0;
+  // chunks of at most NOALLOC_LOG steps, each a continuation of the previous one (the next
+  // chunk's first dt / c_h from the recovered P, as a run continued across mhd_run calls)
+  int left = nsteps;
+  do {
+    const int n = std::min(left, NOALLOC_LOG);
+    CHKRET(start_loop(c, 0, tmax, t, NOALLOC_LOG));
+    for (int s = 0; s < n; ++s) CHKRET(issue_step(c, 0, nullptr, a, dev_log));
+    int got = 0;
+    CHKRET(finish_loop(c, 0, U2, dev_log, n, &t, dt_log ? dt_log + done : nullptr, &got));
+    done += got;
+    left -= n;
+    if (got < n) break;  // stopped: past tmax, or a bad cell
+  } while (left > 0);
+  *time = t;
+  if (steps_done) *steps_done = done;
+  return MHD_OK;
+}
+
 int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int cap, int* nkernels) {
   if (!c || !avg_ms || !nkernels || nsteps < 1) return fail(MHD_ERR_ARG, "bad profile arguments");
   const int nk = launches_per_step(c, mode), nkind = kernel_kinds(mode);
@@ -626,7 +738,7 @@ int mhd_profile_kernels(mhd_ctx* c, int nsteps, int mode, double* avg_ms, int ca
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) { rc = fail(MHD_ERR_HIP, "hipEventCreate"); break; }
   if (rc == MHD_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventRecord");
-  for (int s = 0; s < nsteps && rc == MHD_OK; ++s) rc = issue_step(c, mode, ev.data() + 1 + (size_t)s * nk);
+  for (int s = 0; s < nsteps && rc == MHD_OK; ++s) rc = issue_step(c, mode, ev.data() + 1 + (size_t)s * nk, c->fa, c->dt_log);
   if (rc == MHD_OK && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(MHD_ERR_HIP, "hipEventSynchronize");
   std::vector<double> acc(nkind, 0.0);
   if (rc == MHD_OK)

@@ -71,6 +71,8 @@ void launch_problem_kh(const Geom& g, double gam, double vy_amp, double rrv, con
 void launch_problem_shear(const Geom& g, double gam, const double* ic /* rho0,p0,v0,a,bx0,amp,sigma */,
                           uint64_t seed, const double* x2b, double* P, double* U, hipStream_t s);
 
+// n doubles at p set to 0 by a kernel (a fill node, not a memset, if the stream is captured)
+void launch_zero(double* p, size_t n, hipStream_t s);
 // sc->host_acc := (+inf, 0); or, ring mode, all three slots
 void launch_reset_scalars(StepScalars* sc, int ring, hipStream_t s);
 // min-dt / max-c_h reduction over the interior cells of P.  ring=0: into sc->host_acc;

@@ -5,6 +5,8 @@
 // (ControlTimestep, SetBoundaryCondition, EvaluateCh, GetNumericalFluxD, UpdateConservU,
 // DampPsi, UpdatePrimitvP of src_cpp_kokkos_host).  The fused step kernel lives in
 // mhd_fused.hip.
+#include <algorithm>
+
 #include "mhd_device.hpp"
 #include "mhd_physics.hpp"
 
@@ -214,6 +216,10 @@ __global__ __launch_bounds__(BLK) void k_reduce_prim(Geom g, const double* __res
   }
   unsigned long long* acc = ring ? sc->red[sc->iter % 3] : sc->host_acc;
   block_commit(dtl, chl, bad, acc, sc, ring != 0);
+}
+
+__global__ __launch_bounds__(BLK) void k_zero(double* __restrict__ p, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * BLK + threadIdx.x; t < n; t += (size_t)gridDim.x * BLK) p[t] = 0.0;
 }
 
 __global__ void k_reset_scalars(StepScalars* sc, int ring) {
@@ -538,6 +544,12 @@ void launch_damp(const Geom& g, const double* x1a, const double* x2a, const doub
 void launch_update_prim(const Geom& g, double gam, const double* U, double* P, const StepScalars* sc, hipStream_t s) {
   dim3 grid(cdiv(g.nx, 64), cdiv(g.ny, BLK / 64), g.nz);
   hipLaunchKernelGGL(k_update_prim, grid, dim3(BLK), 0, s, g, gam, U, P, sc);
+}
+
+void launch_zero(double* p, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>(4096, (n + BLK - 1) / BLK);
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(BLK), 0, s, p, n);
 }
 
 void launch_step_end(StepScalars* sc, double cfl, double* dt_log, int ahead, hipStream_t s) {

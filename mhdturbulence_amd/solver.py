@@ -200,6 +200,15 @@ class Box:
         capi.check(self.L.mhd_run(self.h, nsteps, tmax, mode, C.byref(t), _dp(log), C.byref(done)))
         return t.value, log[:done.value].copy(), done.value
 
+    def run_noalloc(self, nsteps: int, time: float = 0.0, tmax: float = 0.0):
+        """run() of the fused step through mhd_run_noalloc (no device allocation; scratch in
+        the library's static device array); returns (time, dt_log, steps_done)."""
+        t = C.c_double(time)
+        log = np.zeros(max(nsteps, 1))
+        done = C.c_int()
+        capi.check(self.L.mhd_run_noalloc(self.h, nsteps, tmax, C.byref(t), _dp(log), C.byref(done)))
+        return t.value, log[:done.value].copy(), done.value
+
     def profile_kernels(self, nsteps: int, mode: int = 0) -> dict:
         """Average ms per launch position of one loop iteration (advances the state)."""
         ms = np.zeros(32)
