@@ -11,8 +11,8 @@ Requests (a shear-layer box of n^3 cells run to time t_end) arrive over time.  T
   next request.
 
 `serve()` replays a request stream against the wall clock and reports per-request latency
-(completion - arrival), p50 / p99 and aggregate cell-updates/s.  With max_boxes=1 the same
-loop is the one-request-at-a-time baseline.
+(completion - arrival), p50 / p99 and aggregate cell-updates/s.  `serve_one_at_a_time()` is
+the baseline: the plain single-box solver taking the same stream one request at a time.
 """
 from __future__ import annotations
 
@@ -93,17 +93,73 @@ def ensemble_box(b: int, seed: int = 20261203) -> tuple[tuple[int, int, int], di
 
 def poisson_stream(rate: float, count: int, seed: int = 20261203, n_range=(32, 256), t_range=(1.0, 6.0),
                    multiple: int = 8) -> list[Request]:
-    """Poisson arrivals at `rate` per second; n log-uniform in n_range rounded to a multiple
-    of `multiple`; t_end uniform in t_range."""
+    """Poisson arrivals at `rate` per second (rate 0: every request at t = 0); n log-uniform
+    in n_range rounded to a multiple of `multiple`; t_end uniform in t_range.  Sizes and end
+    times come from one generator and arrival gaps from another, so a seed gives the same
+    requests at every rate."""
     rng = np.random.default_rng(seed)
+    gaps = np.random.default_rng([seed, 1])
     t = 0.0
     out = []
     for rid in range(count):
-        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        t += gaps.exponential(1.0 / rate) if rate > 0 else 0.0
         n = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
         n = int(max(n_range[0], min(n_range[1], round(n / multiple) * multiple)))
         out.append(Request(rid, n, float(rng.uniform(*t_range)), request_params(rid, seed), t))
     return out
+
+
+def _summary(results: list, wall: float, cell_updates: int, gpu_s: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {
+        "requests": len(results),
+        "wall_s": wall,
+        "gpu_s": gpu_s,
+        "cell_updates": cell_updates,
+        "cell_updates_per_s": cell_updates / wall,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "failed": sum(1 for r in results if r.status),
+        "results": results,
+    }
+
+
+def serve_one_at_a_time(requests: list[Request], cfg: Config, chunk_steps: int = 500, clock=time.perf_counter,
+                        keep_prim: bool = False) -> dict:
+    """The plain single-box solver serving the stream one request at a time, in arrival
+    order: each request gets its own Box (mhd_create, the device problem generator, mhd_run
+    in chunks of chunk_steps with tmax = t_end, diagnostics), as a user of the
+    one-box-per-job reference would run it; the next request starts when it is done."""
+    from .solver import Box
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    t0 = clock()
+    results: list[Result] = []
+    cell_updates = 0
+    gpu_ms = 0.0
+    for r in reqs:
+        now = clock() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        adm = clock() - t0
+        c = Config(**{**cfg.__dict__, "nx": r.n, "ny": r.n, "nz": r.n})
+        box = Box(c)
+        box.generate_grid()
+        box.generate_problem_shear(**r.ic)
+        t, steps = 0.0, 0
+        while True:
+            t, _, done = box.run(chunk_steps, time=t, tmax=r.t_end)
+            gpu_ms += box.last_run_timing()[0]
+            steps += done
+            if done < chunk_steps or t > r.t_end:
+                break
+        diag = box.diagnostics()
+        prim = box.download_prim() if keep_prim else None
+        status = box.status()
+        box.close()
+        cell_updates += steps * r.n ** 3
+        results.append(Result(r.rid, r.n, r.arrival, adm, clock() - t0, steps, t, diag.tolist(), status, prim))
+    return _summary(results, clock() - t0, cell_updates, gpu_ms / 1e3)
 
 
 class Server:
@@ -162,17 +218,4 @@ class Server:
                 results.append(res)
                 if log:
                     log(f"done rid={r.rid} n={r.n} steps={st['steps']} latency={res.latency:.3f}s")
-        wall = clock() - t0
-        lat = np.array([r.latency for r in results])
-        return {
-            "requests": len(results),
-            "wall_s": wall,
-            "gpu_s": gpu_ms / 1e3,
-            "cell_updates": cell_updates,
-            "cell_updates_per_s": cell_updates / wall,
-            "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)),
-            "mean_latency_s": float(lat.mean()),
-            "failed": sum(1 for r in results if r.status),
-            "results": results,
-        }
+        return _summary(results, clock() - t0, cell_updates, gpu_ms / 1e3)

@@ -55,3 +55,53 @@ def test_admission_waits_for_memory():
     srv.close()
     a, b = sorted(out["results"], key=lambda r: r.admitted)
     assert b.admitted >= a.finished - 1e-6
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def _oracle_request(orc, r, stages=2, cfl=0.3, steps=None):
+    """an oracle run of request r: the same box and scheme, stepped until the step that takes
+    time past t_end (src_cpp_kokkos_host/main.cpp:265), or exactly `steps` steps"""
+    o = orc.Oracle(nx=r.n, ny=r.n, nz=r.n, **BASE, stages=stages, cfl=cfl)
+    o.generate_grid()
+    o.generate_problem_shear(**r.ic)
+    t, n = 0.0, 0
+    while (steps is None and t <= r.t_end) or (steps is not None and n < steps):
+        t, _ = o.run(1, t)
+        n += 1
+    return o, t, n
+
+
+@pytest.mark.parametrize("server", ["batched_fp64", "batched_mixed", "one_at_a_time"])
+def test_served_requests_match_oracle(orc, server):
+    """config 5's scheme (two-stage step at CFL 0.3) on small requests served together:
+    each request's final state against an oracle run of that request alone, FP64 at
+    rel L1 <= 1e-10 with the same step count and end time, precision 2 at the mixed bar"""
+    import parity
+    from mhdturbulence_amd.serve import serve_one_at_a_time
+    reqs = [Request(k, n, t_end, request_params(k, 11), 0.0)
+            for k, (n, t_end) in enumerate(((16, 0.09), (24, 0.05), (16, 0.13), (32, 0.04)))]
+    cfg = Config(**BASE, stages=2, cfl=0.3, precision=2 if server == "batched_mixed" else 0)
+    if server == "one_at_a_time":
+        out = serve_one_at_a_time(reqs, cfg, chunk_steps=5, keep_prim=True)
+    else:
+        srv = Server(cfg, max_boxes=4, arena_bytes=1 << 30, chunk_steps=7)
+        out = srv.serve(reqs, keep_prim=True)
+        srv.close()
+    assert out["requests"] == len(reqs) and out["failed"] == 0
+    for res in out["results"]:
+        r = reqs[res.rid]
+        if server == "batched_mixed":
+            o, t, n = _oracle_request(orc, r, steps=res.steps)
+        else:
+            o, t, n = _oracle_request(orc, r)
+            assert res.steps == n and res.sim_time == pytest.approx(t, rel=1e-12)
+        assert res.sim_time > r.t_end and res.status == 0
+        err = parity.rel_l1(o.interior(res.prim), o.interior(o.P), "prim")
+        if server == "batched_mixed":
+            assert err[0] <= 1e-6 and err[5] <= 1e-6 and np.all(err <= 1e-4), err
+        else:
+            assert err.max() <= 1e-10, (r, err)
