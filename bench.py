@@ -33,6 +33,8 @@ the process may use (at most OMP_NUM_THREADS), OMP_PROC_BIND=close, OMP_PLACES=c
 child process run before the GPU is touched; the line names the CPU model and core counts.
 ensemble / serving: the batched and request-serving measurements (skipped by --no-serve);
 ensemble.rk2 repeats the ensemble with the two-stage step (skipped by --no-rk2 too).
+dropin: the reference's own loop through the C++ drop-in layer at 256^3 in separate
+processes (recorded onto the fused kernels, staged, AdvanceSteps; skipped by --no-dropin).
 rk2: the same box advanced by the two-stage Runge-Kutta step at CFL factor 0.3 (the
 configuration with "2-stage RK, CFL 0.3"), beside the reference's one-stage step; both
 report simulated time per wall second (skipped by --no-rk2).
@@ -256,6 +258,42 @@ def ensemble_bench(count: int = 96, steps: int = 20, seed: int = 20261203, stage
     return out
 
 
+def dropin_bench(short: int = 20, long: int = 220) -> dict:
+    """The reference's own loop (src_cpp_kokkos_host/main.cpp:228-267 call sequence, the
+    reference KH problem with gamma 1.4 and B = 0) at 256^3 through the C++ drop-in layer
+    (mhdturbulence_amd/mhd_kh_main_256, separate processes): its step functions recorded
+    and run as the fused kernels (the default), each on its own staged kernels (--staged),
+    and AdvanceSteps (mhd_run, the device loop).  Each variant runs `short` and `long`
+    steps in two processes; ms per step is the difference of the driver's own loop timers
+    over the difference of steps, so the one-time costs of a process (code loading, graph
+    capture, first touch) cancel.  The loop's per-step host work (the dt read-back, printf,
+    dt.log) stays in."""
+    import re
+    import subprocess
+    import tempfile
+    exe = os.path.join(HERE, "mhdturbulence_amd", "mhd_kh_main_256")
+    out = {"workload": "reference KH problem 256^3 through the C++ drop-in", "steps": [short, long]}
+
+    def loop_time(flags, steps, tmp):
+        r = subprocess.run([exe, "--steps", str(steps), *flags], cwd=tmp, capture_output=True, text=True,
+                           timeout=600, check=True)
+        wall = float(re.search(r"sim time \[s\]: (\S+)", r.stdout).group(1))
+        done = int(re.search(r"exiting main loop time=\S+, step=(\d+)", r.stdout).group(1)) + 1
+        return wall, done
+
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, flags in (("reference_loop_fused", []), ("reference_loop_staged", ["--staged"]),
+                            ("advance_steps", ["--fused"])):
+            w0, n0 = loop_time(flags, short, tmp)
+            w1, n1 = loop_time(flags, long, tmp)
+            ms = 1e3 * (w1 - w0) / (n1 - n0)
+            out[name] = {"ms_per_step": ms, "cell_updates_per_s": 256 ** 3 / (ms * 1e-3),
+                         "loop_s": [w0, w1], "steps_done": [n0, n1]}
+    out["reference_loop_fused_over_advance_steps"] = (out["reference_loop_fused"]["ms_per_step"] /
+                                                      out["advance_steps"]["ms_per_step"])
+    return out
+
+
 def rk2_bench(shape, precision: int, mode: int, euler_sim_rate: float, steps: int = 60) -> dict:
     """the two-stage Runge-Kutta step (CFL factor 0.3) on the same box: cell-updates/s (one
     update = both stages), per-kernel ms (both stages of a kernel summed) and simulated time
@@ -298,6 +336,7 @@ def main():
                     help="1: the reference's forward-Euler step; 2: two-stage Runge-Kutta")
     ap.add_argument("--cfl", type=float, default=None, help="CFL factor (default 0.05, or 0.3 with --stages 2)")
     ap.add_argument("--no-rk2", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the C++ drop-in driver runs")
     ap.add_argument("--cpu-baseline-worker", type=int, nargs=4, metavar=("NX", "NY", "NZ", "STEPS"),
                     help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -319,6 +358,13 @@ def main():
     # the CPU baseline runs first, in a child process started before this process touches
     # the GPU
     cpu_line = None
+    dropin = None
+    if not args.no_dropin:
+        log("[bench] the reference loop through the C++ drop-in at 256^3 (separate processes)")
+        dropin = dropin_bench()
+        log(f"[bench] drop-in: reference loop {dropin['reference_loop_fused']['ms_per_step']:.3f} ms/step fused, "
+            f"{dropin['reference_loop_staged']['ms_per_step']:.3f} staged, "
+            f"{dropin['advance_steps']['ms_per_step']:.3f} AdvanceSteps")
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps of {shape_name}")
         cpu_line = cpu_baseline(shape, args.cpu_steps)
@@ -418,6 +464,7 @@ def main():
         r4 = out["serving"]["rate_4x"]
         out["p99_latency_s"] = {"batched_mixed_at_4x_capacity": r4["batched_mixed"]["p99_latency_s"],
                                 "fp64_one_at_a_time_at_4x_capacity": r4["fp64_one_at_a_time"]["p99_latency_s"]}
+    out["dropin"] = dropin
     out["cpu_baseline"] = cpu_line
     print(json.dumps(out), flush=True)
 

@@ -137,6 +137,20 @@ int mhd_update_primitive(mhd_ctx* ctx);
  * applied, and the call still returns MHD_OK. */
 int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done);
 
+/* The fused loop body one reference step at a time, for a caller that issues the reference
+ * loop's calls in their order (the C++ drop-in layer, src_cpp_kokkos_host/main.cpp:229-257)
+ * and wants the fused kernels' speed.  _begin_step is that iteration's ControlTimestep and
+ * EvaluateCh (src_cpp_kokkos_host/mhd.cpp:1511-1611): *dt = cfl*dtmin and *chg of the
+ * current state, read from the bounds the previous step's fused kernels reduced (one small
+ * device-to-host read, which waits for that step); *status (may be NULL) as mhd_status.
+ * _step applies the rest of the iteration (SetBoundaryCondition ... UpdatePrimitvP) with
+ * the fused kernels and that dt and c_h, and returns at once.  Between the calls the state
+ * lives in the fused loop's buffers and P is not updated; the first call of any other entry
+ * point that reads or writes P, U, F or the tiling first brings P and U up to date, as
+ * mhd_run leaves them (so a stretch of steps ends there). */
+int mhd_fused_begin_step(mhd_ctx* ctx, double* dt, double* chg, int* status);
+int mhd_fused_step(mhd_ctx* ctx);
+
 /* Allocation-free variant of mhd_run in mode 0 (the fused step).  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,

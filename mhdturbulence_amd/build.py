@@ -32,6 +32,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmhd_amd.so")
 KH_LIB = os.path.join(PKG, "libmhd_kh.so")
 KH_MAIN = os.path.join(PKG, "mhd_kh_main")
+KH_MAIN_256 = os.path.join(PKG, "mhd_kh_main_256")
 CALIB = os.path.join(PKG, "mhd_calib")
 
 ARCH = os.environ.get("MHD_OFFLOAD_ARCH", "gfx950")
@@ -94,6 +95,10 @@ def build_host(verbose: bool = True) -> str:
     if os.path.exists(main):
         _run([cxx, *flags, *inc, "-o", KH_MAIN, main, f"-L{PKG}", "-lmhd_kh", "-lmhd_amd",
               f"-Wl,-rpath,$ORIGIN"])
+        # the same driver with the grid of the benchmark box compiled in (config.hpp's sizes
+        # are compile-time, so the host layer is compiled into it rather than linked)
+        _run([cxx, *flags, *inc, "-DMHD_NGRID1=256", "-DMHD_NGRID2=256", "-DMHD_NGRID3=256", "-o", KH_MAIN_256,
+              main, *srcs, f"-L{PKG}", "-lmhd_amd", f"-Wl,-rpath,$ORIGIN"])
     if verbose:
         print(f"built {KH_LIB}")
     return KH_LIB

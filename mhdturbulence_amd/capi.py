@@ -17,7 +17,8 @@ EXPORTS = (
     "mhd_device_bytes", "mhd_set_grid", "mhd_generate_grid", "mhd_get_grid", "mhd_upload_prim",
     "mhd_upload_cons", "mhd_download_prim", "mhd_download_cons", "mhd_download_flux",
     "mhd_snapshot_begin", "mhd_snapshot_end", "mhd_control_timestep", "mhd_set_boundary", "mhd_evaluate_ch", "mhd_numerical_flux",
-    "mhd_update_conserv", "mhd_damp_psi", "mhd_update_primitive", "mhd_run", "mhd_run_noalloc",
+    "mhd_update_conserv", "mhd_damp_psi", "mhd_update_primitive", "mhd_run", "mhd_fused_begin_step",
+    "mhd_fused_step", "mhd_run_noalloc",
     "mhd_generate_problem_kh",
     "mhd_generate_problem_shear", "mhd_diagnostics", "mhd_status", "mhd_last_run_timing",
     "mhd_profile_kernels", "mhd_kernel_names",
@@ -92,6 +93,8 @@ def load(path: str | None = None):
         "mhd_damp_psi": (C.c_int, [vp, C.c_double, C.c_double]),
         "mhd_update_primitive": (C.c_int, [vp]),
         "mhd_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, dp, dp, ip]),
+        "mhd_fused_begin_step": (C.c_int, [vp, dp, dp, ip]),
+        "mhd_fused_step": (C.c_int, [vp]),
         "mhd_run_noalloc": (C.c_int, [vp, C.c_int, C.c_double, dp, dp, ip]),
         "mhd_generate_problem_kh": (C.c_int, [vp, C.c_double, C.c_double]),
         "mhd_generate_problem_shear": (C.c_int, [vp, C.POINTER(MhdIcParams)]),

@@ -85,6 +85,9 @@ struct mhd_ctx {
   double* snap_host = nullptr;
   hipEvent_t snap_ready = nullptr, snap_done = nullptr;
   bool snap_recorded = false, snap_pending = false;
+  // the fused loop driven one reference step at a time (mhd_fused_begin_step / _step) holds
+  // the state in U / U2 with P out of date until settle() brings them back
+  bool fz_live = false;
 };
 
 namespace {
@@ -160,9 +163,34 @@ int ensure_fused(mhd_ctx* c) {
 
 int ensure_staging(mhd_ctx* c) { return dalloc(c, &c->staging, mhd::NPRIM * host_cells(c)); }
 
-int set_dev(mhd_ctx* c) {
+int select_dev(mhd_ctx* c) {
   HIPCHK(hipSetDevice(c->dev));
   return MHD_OK;
+}
+
+int read_scalars(mhd_ctx* c, StepScalars* h);
+
+// Ends a stretch of mhd_fused_step calls: the state is put back in U (from U2 after an odd
+// number of one-stage iterations) and P recovered from it, as mhd_run leaves them.
+int settle(mhd_ctx* c) {
+  if (!c->fz_live) return MHD_OK;
+  c->fz_live = false;
+  StepScalars h;
+  CHKRET(read_scalars(c, &h));
+  if (c->cfg.stages != 2 && (h.iter & 1))
+    HIPCHK(hipMemcpyAsync(c->U, c->U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+  mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+// the context's device made current, and P / U up to date (every entry point but the
+// one-step fused pair goes through here)
+int set_dev(mhd_ctx* c) {
+  CHKRET(select_dev(c));
+  return settle(c);
 }
 
 void drop_graph(mhd_ctx* c) {
@@ -648,6 +676,32 @@ int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double*
   return MHD_OK;
 }
 
+// ---- the fused loop body one reference step at a time ----
+int mhd_fused_begin_step(mhd_ctx* c, double* dt, double* chg, int* status) {
+  if (!c || !dt || !chg) return fail(MHD_ERR_ARG, "null argument");
+  if (!c->grid_set) return fail(MHD_ERR_STATE, "grid not set (mhd_set_grid or mhd_generate_grid first)");
+  CHKRET(select_dev(c));
+  if (!c->fz_live) {
+    // a new stretch: the first step's bounds from P, the ghost cells of U, iteration 0
+    CHKRET(ensure_fused(c));
+    CHKRET(start_loop(c, 0, 0.0, 0.0, 0));
+    c->fz_live = true;
+  }
+  StepScalars h;
+  CHKRET(read_scalars(c, &h));
+  *dt = c->cfg.cfl * bits2d(h.red[h.iter % 3][0]);
+  *chg = bits2d(h.red[h.iter % 3][1]);
+  if (status) *status = h.status;
+  return MHD_OK;
+}
+
+int mhd_fused_step(mhd_ctx* c) {
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  CHKRET(select_dev(c));
+  if (!c->fz_live) return fail(MHD_ERR_STATE, "mhd_fused_step without mhd_fused_begin_step");
+  return issue_step(c, 0, nullptr, c->fa, nullptr);
+}
+
 // ---- the allocation-free run (mhd_run_noalloc) ----
 // Its scratch (the fused loop's second state copy, the flux divergence and a dt log chunk)
 // lives in this array, one copy per GPU the library runs on.  Host code never names the
@@ -831,7 +885,7 @@ int mhd_diagnostics(mhd_ctx* c, double* out4) {
 
 int mhd_status(mhd_ctx* c, int* status) {
   if (!c || !status) return fail(MHD_ERR_ARG, "null argument");
-  CHKRET(set_dev(c));
+  CHKRET(select_dev(c));
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   *status = h.status;

@@ -1,5 +1,6 @@
 // boundary.cpp -- boundary_mod over the MI355X library: SetBoundaryCondition fills the
-// ghost cells of P in HBM (src_cpp_kokkos_host/boundary.cpp:750-893 for one tile per axis).
+// ghost cells of P in HBM (src_cpp_kokkos_host/boundary.cpp:750-893 for one tile per axis),
+// or is recorded as part of a step the fused kernels run (see mhd.hpp).
 #include "boundary.hpp"
 
 #include "config.hpp"
@@ -30,8 +31,11 @@ void DeallocateBoundaryVariables(BoundaryArray<double>& Bs_, BoundaryArray<doubl
 void SendRecvBoundary(const BoundaryArray<double>&, BoundaryArray<double>&) {}
 
 void SetBoundaryCondition(hydflux_mod::FieldArray<double>&, BoundaryArray<double>&, BoundaryArray<double>&) {
-  // ghost fill on the device (periodic / reflection / outflow per face)
   hydflux_mod::ensure_device_state();
+  // in a recorded step the fused kernels write the ghost images of the new state themselves
+  if (hydflux_mod::RecordSetBoundaryCondition()) return;
+  hydflux_mod::FlushRecordedStep();
+  // ghost fill on the device (periodic / reflection / outflow per face)
   hydflux_mod::check(mhd_set_boundary(hydflux_mod::device_context()), "mhd_set_boundary");
 }
 

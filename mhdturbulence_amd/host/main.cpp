@@ -2,10 +2,14 @@
 // (GenerateGrid, GenerateProblem, then the step loop at main.cpp:228-267), linked against
 // the drop-in hydflux_mod / boundary_mod layer.
 //
-//   mhd_kh_main [--steps N] [--fused] [--dump FILE] [--tprof]
+//   mhd_kh_main [--steps N] [--fused | --staged] [--dump FILE] [--tprof]
 //
 // --steps N  loop-iteration cap (the reference breaks after step 150)
 // --fused    run the loop body on the device through AdvanceSteps (dt stays in HBM)
+// --staged   the reference loop with every step function on its own staged kernels
+//            (default: the same loop, recorded and run as the fused kernels)
+// --peek N   on step N download Fx right after GetNumericalFluxD(1) (an interruption of
+//            the recorded step: that step runs out on the staged kernels)
 // --dump F   write the final primitive array (host layout [k][j][n][i], f64) to F
 // --tprof    append t-prof.csv rows (time, mix, sqrt(<vy^2>), A exp(Gamma t)) every step
 #include <chrono>
@@ -92,13 +96,15 @@ static void RealTimeAnalysis(FILE* fp) {
 }
 
 int main(int argc, char** argv) {
-  int steps = 150;
+  int steps = 150, peek = -1;
   bool fused = false, tprof = false;
   const char* dump = nullptr;
   for (int a = 1; a < argc; ++a) {
     if (!std::strcmp(argv[a], "--steps") && a + 1 < argc) steps = std::atoi(argv[++a]);
     else if (!std::strcmp(argv[a], "--fused")) fused = true;
+    else if (!std::strcmp(argv[a], "--staged")) SetStepPath(StepPath::staged);
     else if (!std::strcmp(argv[a], "--tprof")) tprof = true;
+    else if (!std::strcmp(argv[a], "--peek") && a + 1 < argc) peek = std::atoi(argv[++a]);
     else if (!std::strcmp(argv[a], "--dump") && a + 1 < argc) dump = argv[++a];
   }
   std::printf("setup grids and fields\n");
@@ -129,6 +135,12 @@ int main(int argc, char** argv) {
       SetBoundaryCondition(P, Bs, Br);
       EvaluateCh();
       GetNumericalFluxD(1, G, P, Fx);
+      if (step == peek) {
+        Fx.d2h();
+        double s1 = 0.0;
+        for (int j = js; j <= je; ++j) s1 += Fx(mden, ks, j, is + 1);
+        std::printf("peek step %d: sum of the x mass flux on a line %.17g\n", step, s1);
+      }
       GetNumericalFluxD(2, G, P, Fy);
       GetNumericalFluxD(3, G, P, Fz);
       UpdateConservU(G, Fx, Fy, Fz, U);

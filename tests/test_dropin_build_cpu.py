@@ -60,3 +60,61 @@ def test_reference_style_loop_compiles_and_links(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert exe.exists()
+
+
+REF_MAIN = "/root/reference/src_cpp_kokkos_host/main.cpp"
+
+WRAP = r"""
+#include <chrono>
+#include <cstdio>
+
+#include "config.hpp"
+#include "mhd.hpp"
+#include "boundary.hpp"
+
+using namespace resolution_mod;
+using namespace hydflux_mod;
+using namespace boundary_mod;
+
+// names the loop takes from the reference's runtime rather than from its step
+static const int myid_w = 0;            // one process (mpi_config_mod::myid_w)
+static const bool usualoutput = false;  // src_cpp_kokkos_host/main.cpp:195
+static void Output(bool) {}             // the snapshot writer of output.cpp
+
+int main() {
+  AllocateHydroVariables(G, U, Fx, Fy, Fz, P);
+  AllocateBoundaryVariables(Bs, Br);
+  int step = 0;
+  FILE* fpdt = nullptr;
+@LOOP@
+  P.d2h();
+  DeallocateBoundaryVariables(Bs, Br);
+  DeallocateHydroVariables(G, U, Fx, Fy, Fz, P);
+  return step > 0 ? 0 : 1;
+}
+"""
+
+
+def test_reference_loop_text_compiles_against_dropin(tmp_path):
+    """src_cpp_kokkos_host/main.cpp:228-267, the reference's step loop as its file holds it
+    (read at test time, never copied into the repository), compiles and links unchanged
+    against the drop-in headers and libraries"""
+    if not os.path.exists(REF_MAIN):
+        pytest.skip("the reference sources are not on this machine")
+    libs = [os.path.join(PKG, n) for n in ("libmhd_kh.so", "libmhd_amd.so")]
+    if not all(os.path.exists(p) for p in libs):
+        pytest.skip("libraries not built (python -m mhdturbulence_amd.build)")
+    lines = open(REF_MAIN).read().splitlines()
+    loop = "\n".join(lines[227:267])
+    assert loop.lstrip().startswith("for (step=0;step<stepmax;step++){"), loop[:80]
+    for call in ("ControlTimestep(G);", "SetBoundaryCondition(P,Bs,Br);", "EvaluateCh();", "UpdateConservU(G,Fx,Fy,Fz,U);",
+                 "DampPsi(G,U);", "UpdatePrimitvP(U,P);", "time_sim += dt;"):
+        assert call in loop, call
+    src = tmp_path / "ref_loop.cpp"
+    src.write_text(WRAP.replace("@LOOP@", loop))
+    exe = tmp_path / "ref_loop"
+    r = subprocess.run([shutil.which("g++") or "g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.join(PKG, "host"), str(src), "-o", str(exe), "-L", PKG, "-lmhd_kh", "-lmhd_amd",
+                        f"-Wl,-rpath,{PKG}"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert exe.exists()

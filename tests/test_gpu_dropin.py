@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "mhdturbulence_amd", "mhd_kh_main")
 N = 150
-STEPS = 4
+STEPS = 5
 
 
 @pytest.fixture(scope="module")
@@ -26,8 +26,11 @@ def oracle_state(oracle_lib):
     return o, t, log
 
 
-@pytest.mark.parametrize("flag", [[], ["--fused"]])
+@pytest.mark.parametrize("flag", [[], ["--staged"], ["--fused"], ["--peek", "2"]])
 def test_kh_main_matches_oracle(tmp_path, oracle_state, flag):
+    """the reference loop through the drop-in: recorded and run as the fused kernels (no
+    flag), every function on its staged kernels (--staged), AdvanceSteps (--fused), and a
+    recorded step interrupted by a flux download that runs out staged (--peek)"""
     o, t, log = oracle_state
     dump = tmp_path / "P.bin"
     r = subprocess.run([EXE, "--steps", str(STEPS), "--dump", str(dump), *flag], cwd=tmp_path, capture_output=True,
