@@ -300,14 +300,13 @@ int valid_box(mhd_batch* b, int id) {
   return MHD_OK;
 }
 
-// dt / c_h of the box's current P into ring slot 0 and the ghost cells of U0
+// dt / c_h of the box's current P into ring slot 0 (the fused kernels read no ghost cell of U0)
 int box_prologue(mhd_batch* b, BatchBox& x, StepScalars* sc, double tmax) {
   StepScalars h{};
   h.tmax = tmax;
   h.red[0][0] = h.red[1][0] = h.red[2][0] = mhd::DT_INIT_BITS;
   BHIP(hipMemcpyAsync(sc, &h, sizeof h, hipMemcpyHostToDevice, b->stream));
   mhd::launch_reduce_prim(x.g, x.P, x.grid[0], x.grid[2], x.grid[4], sc, 1, b->stream);
-  mhd::launch_boundary(x.g, b->base.bc, x.U0, mhd::NCONS, b->stream);
   BHIP(hipGetLastError());
   return MHD_OK;
 }

@@ -580,7 +580,7 @@ static int launch_kind(const mhd_ctx* c, int mode, int q) {
 }
 
 // the loop scalars of a run starting at `time` with a device dt log of dt_cap entries, and
-// (mode 0) the first iteration's dt / c_h from P and the ghost cells of U; allocates nothing
+// (mode 0) the first iteration's dt / c_h from P; allocates nothing
 static int start_loop(mhd_ctx* c, int mode, double tmax, double time, int dt_cap) {
   double tt[2] = {time, tmax};
   HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
@@ -591,9 +591,9 @@ static int start_loop(mhd_ctx* c, int mode, double tmax, double time, int dt_cap
   mhd::launch_reset_scalars(c->sc, 1, c->stream);
   if (mode == 0) {
     // the first iteration's dt and c_h from P (ControlTimestep / EvaluateCh on the state as
-    // given), and the ghost cells of U, which the fused kernels read
+    // given); the fused kernels read no ghost cell of U (they take boundary images from the
+    // interior cells)
     mhd::launch_reduce_prim(c->g, c->P, c->grid[0], c->grid[2], c->grid[4], c->sc, 1, c->stream);
-    mhd::launch_boundary(c->g, c->cfg.bc, c->U, mhd::NCONS, c->stream);
   }
   HIPCHK(hipGetLastError());
   return MHD_OK;
@@ -682,7 +682,7 @@ int mhd_fused_begin_step(mhd_ctx* c, double* dt, double* chg, int* status) {
   if (!c->grid_set) return fail(MHD_ERR_STATE, "grid not set (mhd_set_grid or mhd_generate_grid first)");
   CHKRET(select_dev(c));
   if (!c->fz_live) {
-    // a new stretch: the first step's bounds from P, the ghost cells of U, iteration 0
+    // a new stretch: the first step's bounds from P, iteration 0
     CHKRET(ensure_fused(c));
     CHKRET(start_loop(c, 0, 0.0, 0.0, 0));
     c->fz_live = true;

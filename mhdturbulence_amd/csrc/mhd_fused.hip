@@ -211,26 +211,44 @@ __device__ __forceinline__ int ghost_src(int bc_lo, int bc_hi, int side, int gg,
   return side ? hi : lo;
 }
 
-// the ghost cells whose boundary image is interior cell (i,j,k) get its new state
-__device__ __forceinline__ void write_images(double* __restrict__ Uo, const Geom& g, const int* bc, int i, int j,
-                                             int k, const double* u) {
-  const int n3[3] = {g.nx, g.ny, g.nz};
-  const int idx[3] = {i, j, k};
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    const int c = idx[ax], lo = NGH, hi = NGH + n3[ax] - 1;
-    if (c > lo + 1 && c < hi - 1) continue;
-    for (int p = 0; p < 4; ++p) {
-      const int side = p >> 1, gg = p & 1;
-      bool neg;
-      if (ghost_src(bc[2 * ax], bc[2 * ax + 1], side, gg, lo, hi, neg) != c) continue;
-      int t[3] = {i, j, k};
-      t[ax] = side ? hi + 1 + gg : gg;
-      const unsigned oc = boff(g.cell(t[2], t[1], t[0]));
-#pragma unroll
-      for (int m = 0; m < NCONS; ++m) stc(Uo, g.vstride, m, oc, (neg && m == 1 + ax) ? -u[m] : u[m]);
-    }
-  }
+// The fused loop never writes ghost cells.  A load of a ghost row, column or plane reads the
+// interior cell whose image that ghost is and, at a reflecting wall, negates the normal
+// momentum: exactly the values the boundary fill would have stored there.  c in
+// [lo - 2, hi + 3] along an axis of n interior cells; the index hi + 3 (n + 4, allocated but
+// never a cell) maps like hi + 2, and no result depends on it.
+__device__ __forceinline__ int image_of(int c, int n, int bc_lo, int bc_hi, bool& neg) {
+  const int lo = NGH, hi = NGH + n - 1;
+  neg = false;
+  if (c >= lo && c <= hi) return c;
+  const int side = c > hi;
+  return ghost_src(bc_lo, bc_hi, side, side ? min(c - hi - 1, 1) : c, lo, hi, neg);
+}
+
+// v, negated when neg (a wave-uniform flag): one XOR of the sign bit, no branch
+__device__ __forceinline__ double negate_if(double v, bool neg) {
+  const unsigned long long m = neg ? 0x8000000000000000ull : 0ull;
+  return __longlong_as_double((long long)(d2bits(v) ^ m));
+}
+
+// conserved state of the cell at (k, j, i) as seen through the boundary images of axis AX
+// (0 x: i may be a ghost column; 1 y: j a ghost row; 2 z: k a ghost plane)
+template <int AX>
+__device__ __forceinline__ void load_image(const double* __restrict__ U, const Geom& g, const int* bc, int k, int j,
+                                           int i, double* u) {
+  bool neg;
+  if constexpr (AX == 0) i = image_of(i, g.nx, bc[0], bc[1], neg);
+  if constexpr (AX == 1) j = image_of(j, g.ny, bc[2], bc[3], neg);
+  if constexpr (AX == 2) k = image_of(k, g.nz, bc[4], bc[5], neg);
+  load_raw(U, g.vstride, g.cell(k, j, i), u);
+  u[mrv1 + AX] = negate_if(u[mrv1 + AX], neg);
+}
+
+template <int AX, typename R>
+__device__ __forceinline__ void load_prim_image(const double* __restrict__ U, const Geom& g, const int* bc, int k,
+                                                int j, int i, double gam, R* q) {
+  double u[NCONS];
+  load_image<AX>(U, g, bc, k, j, i, u);
+  recover<R>(u, gam, q);
 }
 
 // LDS window ring of one FP64 march wave (xy and z), in values of R: 4 slots of NPRIM x 64
@@ -396,7 +414,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
       const int fc = NGH + (s_first + fi) * 64;  // the face between cells fc-1 and fc
       R q[4][NPRIM];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) load_prim<R>(U, vs, g.cell(k, j0 + r, fc - 2 + c), a.gam, q[c]);
+      for (int c = 0; c < 4; ++c) load_prim_image<0, R>(U, g, a.bc, k, j0 + r, fc - 2 + c, a.gam, q[c]);
       R ql[NPRIM], qr[NPRIM], unused[NPRIM];
       cell_states<R>(q[0], q[1], q[2], ql, unused);
       cell_states<R>(q[1], q[2], q[3], unused, qr);
@@ -435,7 +453,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     const int fc = side ? ilast + 1 : i0;  // the face between cells fc-1 and fc
     R q[4][NPRIM];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) load_prim<R>(U, vs, g.cell(kr, j0 + r, fc - 2 + c), a.gam, q[c]);
+    for (int c = 0; c < 4; ++c) load_prim_image<0, R>(U, g, a.bc, kr, j0 + r, fc - 2 + c, a.gam, q[c]);
     R ql[NPRIM], qr[NPRIM], unused[NPRIM];
     cell_states<R>(q[0], q[1], q[2], ql, unused);
     cell_states<R>(q[1], q[2], q[3], unused, qr);
@@ -462,9 +480,9 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   };
   R Pm[NPRIM], P0[NPRIM], Pp[NPRIM], upprev[NPRIM];
   double Fyprev[NCONS];
-  load_prim<R>(U, vs, g.cell(k, j0 - 2, i), a.gam, Pm);
-  load_prim<R>(U, vs, g.cell(k, j0 - 1, i), a.gam, P0);
-  load_prim<R>(U, vs, g.cell(k, j0, i), a.gam, Pp);
+  load_prim_image<1, R>(U, g, a.bc, k, j0 - 2, i, a.gam, Pm);
+  load_prim_image<1, R>(U, g, a.bc, k, j0 - 1, i, a.gam, P0);
+  load_prim_image<1, R>(U, g, a.bc, k, j0, i, a.gam, Pp);
   const Cols Uc = make_cols(U, vs);
   if (coop) {
     if (lane == 0 || lane == 63) {
@@ -491,7 +509,10 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   // latency; only computed values cross the loop edge, so no load in flight is copied.
   for (int j = j0 - 1; j <= j1; ++j) {
     double un[NCONS];
-    load_raw(Uc, g.cell(k, j + 2, i), un);  // row j+2 <= ny+4 < jtot
+    // row j+2, or the row it is the image of (made scalar: j is uniform over the wave)
+    bool neg_n;
+    const int jn = image_of(__builtin_amdgcn_readfirstlane(j + 2), g.ny, a.bc[2], a.bc[3], neg_n);
+    load_raw(Uc, g.cell(k, jn, i), un);
     const int row = j - 1;  // the cell row completed in this pass
     const bool out_row = row >= j0;
     double ax[NCONS];
@@ -583,6 +604,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     // row j+2 enters before this pass's stores are issued: a wait for a load behind a
     // pending store would drain the store too (one counter for both on this architecture)
     R Pn[NPRIM];
+    un[mrv2] = negate_if(un[mrv2], neg_n);
     recover_in_place<R>(un, a.gam, Pn);
     if constexpr (PRING) wr(sm, Pn);
     if (coop && (lane == 0 || lane == 63)) {
@@ -630,7 +652,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   __shared__ R xring[xy_ring_len<R>()];
   __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
   __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
   xy_body<EFLUX, R, WIDE, ACC32>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
                           smem + wv * xy_wave_lds<R>(a.sw, a.ly), xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0),
                           xedgeP, xedgeF, wv);
@@ -686,9 +708,6 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
   const int kl = k0 + min(lane, k1 - k0 - 1);
   const double dz_l = pin_lanes(a.x3a[kl + 1] - a.x3a[kl]);
   const double zinv_l = pin_lanes(1.0 / (a.x3a[kl + 1] - a.x3a[kl]));
-  // ghost images exist only for cells within two of a face: a wave whose x-strip and rows
-  // are away from the x and y faces skips write_images except on the first/last two planes
-  const bool img_xy = s * SW <= 1 || (s + 1) * SW - 1 >= g.nx - 2 || jg * SPW <= 1 || jg * SPW + SPW - 1 >= g.ny - 2;
 
   // FP64: the window planes and the previous pass's upper interface state live in this
   // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold planes k-1, k, k+1
@@ -708,8 +727,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) ring[(slot * NPRIM + n) * 64 + lane] = q[n];
   };
-  load_prim<R>(U, vs, g.cell(k0 - 2, j, i), a.gam, wm);
-  load_prim<R>(U, vs, g.cell(k0 - 1, j, i), a.gam, w0);
+  load_prim_image<2, R>(U, g, a.bc, k0 - 2, j, i, a.gam, wm);
+  load_prim_image<2, R>(U, g, a.bc, k0 - 1, j, i, a.gam, w0);
   const Cols Uc = make_cols(U, vs), Uoc = make_cols(Uo, vs);
   {
     double u0[NCONS];
@@ -740,7 +759,10 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     const int kk = k - 1;  // the cell updated in this pass
     const bool upd = kk >= k0 && active;
     double un[NCONS];
-    load_raw(Uc, g.cell(k + 2, j, i), un);  // plane k+2 <= nz+4 < ktot
+    // plane k+2, or the plane it is the image of (made scalar: k is uniform over the wave)
+    bool neg_n;
+    const int kn = image_of(__builtin_amdgcn_readfirstlane(k + 2), g.nz, a.bc[4], a.bc[5], neg_n);
+    load_raw(Uc, g.cell(kn, j, i), un);
     double uo[NCONS], ac[NCONS];
     const unsigned oc = boff(g.cell(kk, j, i));
     const double dza = lane_row(dz_l, max(kk - k0, 0));
@@ -777,6 +799,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     if (k >= k0) riemann_flux<2, EFLUX, R>(upprev, dn, chg, hchg, Fz);
     // plane k+2 enters before this pass's stores are issued (see the xy march)
     R Pn[NPRIM];
+    un[mrv3] = negate_if(un[mrv3], neg_n);
     recover_in_place<R>(un, a.gam, Pn);
     if constexpr (PRING) {
       wr(sm, Pn);
@@ -803,7 +826,6 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
         for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (un0[m] + u[m]);
       }
       store_cols(Uoc, oc, u);
-      if (img_xy || kk <= NGH + 1 || kk >= NGH + g.nz - 2) write_images(Uo, g, a.bc, i, j, kk, u);
       // the next iteration's ControlTimestep / EvaluateCh bounds of this cell
       if constexpr (reduce) {
         double q[NPRIM];
@@ -846,7 +868,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
   // LDS: FP64 window ring (96 KiB) + raw ring (60 KiB) per block; FP32 raw ring only
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
   z_body<EFLUX, R, STAGE, ACC32>(a, blockIdx.x * WAVES + wv, threadIdx.x & 63,
                                  zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                           zraw + wv * Z_RAW);
@@ -865,7 +887,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
   __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int2 e = tab[blockIdx.x];
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
   R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
   if (xy_wide(a))
@@ -880,7 +902,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
   const FusedArgs& a = boxes[__builtin_amdgcn_readfirstlane(e.x)];
   __shared__ R zring[sizeof(R) == 8 ? WAVES * WIN_RING : 1];
   __shared__ double zraw[WAVES * Z_RAW];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
   z_body<EFLUX, R, STAGE, ACC32>(a, e.y * WAVES + wv, threadIdx.x & 63, zring + (sizeof(R) == 8 ? wv * WIN_RING : 0),
                           zraw + wv * Z_RAW);
 }

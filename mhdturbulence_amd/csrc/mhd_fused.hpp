@@ -6,12 +6,13 @@
 //          in LDS or, for FP32 full-wave strips, from the neighbouring wavefronts of the
 //          block; it marches in y keeping a 3-row window, in LDS for FP64)
 //   z    : z-face fluxes marching in z, then U_new = U - dt*(acc + dFz/dz), psi damping,
-//          primitive recovery of U_new, the next iteration's dt / c_h reduction, and the
-//          ghost cells of U_new (periodic / reflecting / outflow images)
+//          primitive recovery of U_new and the next iteration's dt / c_h reduction
 //
 // Only the 10 conserved variables stay resident (two ping-pong copies); primitives are
 // recovered from U on load, as UpdatePrimitvP (src_cpp_kokkos_host/mhd.cpp:1447-1509)
-// would have stored them.
+// would have stored them.  No kernel writes a ghost cell: a load of a ghost row, column or
+// plane reads the interior cell whose periodic / reflecting / outflow image it is (normal
+// momentum negated at a reflecting wall), the values SetBoundaryCondition would store.
 #pragma once
 
 #include "mhd_device.hpp"
