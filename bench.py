@@ -61,21 +61,19 @@ HBM_PEAK_GBS = 8000.0
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1356751.4 * 1024 * 2.0 + 1310725.0 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1374071.2 * 1024 * 2.0 + 1392898.2 * 1024 * 1.0,
-    (256, 0, 1, "fused_xy"): 732328.0 * 1024 * 2.0 + 1310725.7 * 1024 * 1.0,
-    (256, 0, 1, "fused_z_update"): 1558325.3 * 1024 * 2.0 + 1393845.1 * 1024 * 1.0,
-    (256, 0, 2, "fused_xy"): 732336.6 * 1024 * 2.0 + 655380.7 * 1024 * 1.0,
-    (256, 0, 2, "fused_z_update"): 1281360.4 * 1024 * 2.0 + 1393775.9 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1323736.0 * 1024 * 2.0 + 1310723.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373636.0 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 2, "fused_xy"): 719814.0 * 1024 * 2.0 + 655363.0 * 1024 * 1.0,
+    (256, 0, 2, "fused_z_update"): 1280504.0 * 1024 * 2.0 + 1311744.0 * 1024 * 1.0,
 }
 
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
 # MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
 MEASURED_VALU = {
-    (256, 0, 0, "fused_xy"): 391.1e6 * 64,
-    (256, 0, 0, "fused_z_update"): 249.2e6 * 64,
-    (256, 0, 1, "fused_xy"): 328.6e6 * 64,
-    (256, 0, 1, "fused_z_update"): 241.5e6 * 64,
+    (256, 0, 0, "fused_xy"): 390.3e6 * 64,
+    (256, 0, 0, "fused_z_update"): 240.1e6 * 64,
+    (256, 0, 2, "fused_xy"): 329.0e6 * 64,
+    (256, 0, 2, "fused_z_update"): 241.4e6 * 64,
 }
 # every SIMD issuing one wave64 VALU instruction per 4 clocks: 256 CUs x 4 SIMDs x 64 lanes
 # / 4 clocks x 2.4 GHz
