@@ -573,7 +573,8 @@ int mhd_batch_finish_box(mhd_batch* b, int id, double* diag4, double* P_host) {
   // one-stage iterations leave the state in U1 after an odd number of them (two-stage ones
   // end in U0)
   if (b->base.stages != 2 && (h.iter & 1)) {
-    BHIP(hipMemcpyAsync(x.U0, x.U1, mhd::NCONS * x.g.vstride * 8, hipMemcpyDeviceToDevice, b->stream));
+    mhd::launch_copy_interior(x.g, x.U1, x.U0, mhd::NCONS, b->stream);
+    BHIP(hipGetLastError());
     // the state is in U0 again; a later step must read U0 first
     int z = 0;
     BHIP(hipMemcpyAsync(&b->d_sc[id].iter, &z, sizeof(int), hipMemcpyHostToDevice, b->stream));

@@ -177,9 +177,7 @@ int settle(mhd_ctx* c) {
   c->fz_live = false;
   StepScalars h;
   CHKRET(read_scalars(c, &h));
-  if (c->cfg.stages != 2 && (h.iter & 1))
-    HIPCHK(hipMemcpyAsync(c->U, c->U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
-                          c->stream));
+  if (c->cfg.stages != 2 && (h.iter & 1)) mhd::launch_copy_interior(c->g, c->U2, c->U, mhd::NCONS, c->stream);
   mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -624,11 +622,10 @@ static int finish_loop(mhd_ctx* c, int mode, const double* U2, const double* dev
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   if (mode == 0) {
-    // one-stage iterations leave the state in U2 after an odd number of them; P is
-    // recovered from it (two-stage iterations end in U)
-    if (!two_stage(c) && (h.iter & 1))
-      HIPCHK(hipMemcpyAsync(c->U, U2, mhd::NCONS * c->g.vstride * sizeof(double), hipMemcpyDeviceToDevice,
-                            c->stream));
+    // one-stage iterations leave the state in U2 after an odd number of them; its interior
+    // goes back to U (whose ghost cells the loop leaves as they were, as the reference's
+    // loop does) and P is recovered from it (two-stage iterations end in U)
+    if (!two_stage(c) && (h.iter & 1)) mhd::launch_copy_interior(c->g, U2, c->U, mhd::NCONS, c->stream);
     mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -73,6 +73,10 @@ void launch_problem_shear(const Geom& g, double gam, const double* ic /* rho0,p0
 
 // n doubles at p set to 0 by a kernel (a fill node, not a memset, if the stream is captured)
 void launch_zero(double* p, size_t n, hipStream_t s);
+// the interior cells of nv variables of src copied into dst, whose ghost cells are left as
+// they are (the fused loop hands its state back this way: the reference's loop never
+// writes the ghost cells of U)
+void launch_copy_interior(const Geom& g, const double* src, double* dst, int nv, hipStream_t s);
 // sc->host_acc := (+inf, 0); or, ring mode, all three slots
 void launch_reset_scalars(StepScalars* sc, int ring, hipStream_t s);
 // min-dt / max-c_h reduction over the interior cells of P.  ring=0: into sc->host_acc;

@@ -384,6 +384,17 @@ __global__ __launch_bounds__(BLK) void k_update_prim(Geom g, double gam, const d
   for (int n = 0; n < NPRIM; ++n) P[n * g.vstride + c] = q[n];
 }
 
+// interior cells of nv variables from src to dst; the ghost cells of dst keep their values
+__global__ __launch_bounds__(BLK) void k_copy_interior(Geom g, const double* __restrict__ src,
+                                                       double* __restrict__ dst, int nv) {
+  const int i = NGH + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = NGH + blockIdx.y * (BLK / 64) + (threadIdx.x >> 6);
+  const int k = NGH + blockIdx.z;
+  if (i >= NGH + g.nx || j >= NGH + g.ny) return;
+  const size_t c = g.cell(k, j, i);
+  for (int m = 0; m < nv; ++m) dst[m * g.vstride + c] = src[m * g.vstride + c];
+}
+
 // ahead = 0: the next iteration reduces into slot iter % 3 (staged loop);
 // ahead = 1: the next iteration reads slot iter % 3 and accumulates into (iter+1) % 3 (fused loop).
 __global__ void k_step_end(StepScalars* sc, double cfl, double* dt_log, int ahead) {
@@ -544,6 +555,11 @@ void launch_damp(const Geom& g, const double* x1a, const double* x2a, const doub
 void launch_update_prim(const Geom& g, double gam, const double* U, double* P, const StepScalars* sc, hipStream_t s) {
   dim3 grid(cdiv(g.nx, 64), cdiv(g.ny, BLK / 64), g.nz);
   hipLaunchKernelGGL(k_update_prim, grid, dim3(BLK), 0, s, g, gam, U, P, sc);
+}
+
+void launch_copy_interior(const Geom& g, const double* src, double* dst, int nv, hipStream_t s) {
+  dim3 grid(cdiv(g.nx, 64), cdiv(g.ny, BLK / 64), g.nz);
+  hipLaunchKernelGGL(k_copy_interior, grid, dim3(BLK), 0, s, g, src, dst, nv);
 }
 
 void launch_zero(double* p, size_t n, hipStream_t s) {

@@ -347,6 +347,31 @@ def test_run_continues_across_calls(orc):
     assert np.all(rel_l1(o.interior(b.download_cons()), o.interior(o.U)) <= TOL_L1)
 
 
+@pytest.mark.parametrize("mode,steps", [(0, 3), (0, 4), (1, 3)])
+def test_loop_leaves_ghost_cells_of_u(orc, mode, steps):
+    """The reference's loop writes only interior cells of U (src_cpp_kokkos_host/mhd.cpp:1431,
+    1625; SetBoundaryCondition fills P): after a run, U's ghost cells hold what was uploaded,
+    bitwise, as the oracle's do, after an odd number of fused steps too (the state then
+    comes back from the second buffer, interior only)"""
+    o, b = pair(orc, n=(24, 16, 12), problem=dict(bx0=0.1), gam=5.0 / 3.0, energy_flux=1)
+    U0 = o.U.copy()
+    idx = np.arange(U0.size).reshape(U0.shape)
+    ghost = np.ones(U0.size, bool)
+    ghost[o.interior(idx).ravel()] = False
+    ghost = ghost.reshape(U0.shape)
+    # ghost values no step computes, so a buffer's stale or zeroed ghosts cannot pass
+    U_up = U0.copy()
+    U_up[ghost] = 7.0 + 1e-3 * np.arange(ghost.sum())
+    b.upload_cons(U_up)
+    b.run(steps, mode=mode)
+    o.run(steps)
+    U = b.download_cons()
+    say(f"mode {mode} steps {steps}: {ghost.sum()} ghost values")
+    assert np.array_equal(o.U[ghost], U0[ghost])
+    assert np.array_equal(U[ghost], U_up[ghost])
+    assert np.all(rel_l1(o.interior(U), o.interior(o.U)) <= TOL_L1)
+
+
 def test_diagnostics_match(orc):
     o, b = pair(orc, n=(32, 32, 16), problem=dict(bx0=0.1, a=0.05, amp=0.03), energy_flux=1)
     o.run(20)
