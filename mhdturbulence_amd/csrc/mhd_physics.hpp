@@ -389,6 +389,176 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
   nf.sc = (L.fsc + (sclst - scl) * msl) * maxs1 + (Rt.fsc + (scrst - scr) * msr) * mins1;
 }
 
+// Upwind-side HLLD (MHD_HLLD_NEAR=1, default).  The reference's branch-free flux
+// (KH/mhd.cpp:381-415) is a left-side term times max(0, sign S_M) plus a right-side term
+// times -min(0, sign S_M): for every face one side's term survives, the left one for
+// S_M >= +0 and the right one otherwise, and the other is multiplied by an exact zero.
+// This form picks that side once S_M is known and forms only its star state, double-star
+// state, physical flux and flux terms, with the reference's expressions.  The other side
+// contributes what the double-star average reads from both sides: its star density,
+// transverse velocity and transverse field.  The result is the surviving term of the
+// reference's sum; the right star energy is grouped as the left one is (KH/mhd.cpp:308-310
+// against :329-331), a difference of round-off.
+#ifndef MHD_HLLD_NEAR
+#define MHD_HLLD_NEAR 1
+#endif
+
+// One interface state in the rotated frame, as HLLD reads it before the side is known
+template <typename R>
+struct RPrim {
+  R rho, vn, vv, vw, bn, bv, bw, E, pt, X, eb, cf, psi;  // eb: |B|^2 or v.B (energy flux)
+};
+
+template <int D, int EFLUX, typename R>
+__device__ __forceinline__ void prim_to_rprim(const R* q, R rhoinv, RPrim<R>& s) {
+  constexpr int DF = (D + 1) % 3, DB = (D + 2) % 3;
+  const R rho = q[nden];
+  const R vsq = q[nve1] * q[nve1] + q[nve2] * q[nve2] + q[nve3] * q[nve3];
+  const R bsq = q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3];
+  const R bn = q[nbm1 + D];
+  s.rho = rho;
+  s.vn = q[nve1 + D];
+  s.vv = q[nve1 + DF];
+  s.vw = q[nve1 + DB];
+  s.bn = bn;
+  s.bv = q[nbm1 + DF];
+  s.bw = q[nbm1 + DB];
+  s.E = rho * q[nene] + R(0.5) * rho * vsq + R(0.5) * bsq;
+  s.pt = q[npre] + bsq / R(2);
+  s.X = q[nst];
+  if constexpr (EFLUX == 0) s.eb = bsq;
+  else s.eb = q[nve1] * q[nbm1] + q[nve2] * q[nbm2] + q[nve3] * q[nbm3];
+  const R css = q[ncsp] * q[ncsp];
+  const R cts = css + bsq * rhoinv;
+  s.cf = sq((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  s.psi = q[nbps];
+}
+
+template <typename R>
+__device__ __forceinline__ R pick(bool right, R l, R r) {
+  return right ? r : l;
+}
+
+template <typename R>
+__device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt, RFlux<R>& nf) {
+  const R eps = R(1.0e-30);
+  const R tiny = R(1.0e-20);
+  const R rol = L.rho, vxl = L.vn, ptl = L.pt;
+  const R ror = Rt.rho, vxr = Rt.vn, ptr = Rt.pt;
+
+  // wave speeds, contact speed and star pressure (KH/mhd.cpp:249-266)
+  const R cfmax = fmax(L.cf, Rt.cf);
+  const R sl = fmin(vxl, vxr) - cfmax;
+  const R sr = fmax(vxl, vxr) + cfmax;
+  const R sdl = fmin(-tiny, sl - vxl);
+  const R sdr = fmax(tiny, sr - vxr);
+  const R rosdl = rol * sdl;
+  const R rosdr = ror * sdr;
+  const R temp = dv(R(1), rosdr - rosdl);
+  const R sm = (rosdr * vxr - rosdl * vxl - ptr + ptl) * temp;
+  const R sdml = fmin(-tiny, sl - sm);
+  const R sdmr = fmax(tiny, sr - sm);
+  const R ptst = (rosdr * ptl - rosdl * ptr + rosdl * rosdr * (vxr - vxl)) * temp;
+
+  // star-state denominators of both sides and their reciprocals (KH/mhd.cpp:268-276, 304-312)
+  const R bxs = R(0.5) * (L.bn + Rt.bn);
+  const R bxsq = bxs * bxs;
+  const R tfl = rosdl * sdml - bxsq;
+  const R sgl = copysign1(fabs(tfl) - eps);
+  const R tfr = rosdr * sdmr - bxsq;
+  const R sgr = copysign1(fabs(tfr) - eps);
+  R itfl, isdml, itfr, isdmr;
+  recip4(tfl + fmin(R(0), sgl), sdml, tfr + fmin(R(0), sgr), sdmr, itfl, isdml, itfr, isdmr);
+
+  // of both star states, what the double-star average reads (KH/mhd.cpp:268-333)
+  struct Star {
+    R maxs, mins, ro, vy, vz, by, bz;
+  };
+  auto star = [&](const RPrim<R>& P, R sd, R sdm, R rosd, R itf, R isdm, R sg) {
+    Star s;
+    s.maxs = fmax(R(0), sg);
+    s.mins = fmin(R(0), sg);
+    R t = bxs * (sd - sdm) * itf;
+    s.ro = s.maxs * (rosd * isdm) - s.mins * P.rho;
+    s.vy = s.maxs * (P.vv - P.bv * t) - s.mins * P.vv;
+    s.vz = s.maxs * (P.vw - P.bw * t) - s.mins * P.vw;
+    t = (rosd * sd - bxsq) * itf;
+    s.by = s.maxs * (P.bv * t) - s.mins * P.bv;
+    s.bz = s.maxs * (P.bw * t) - s.mins * P.bw;
+    return s;
+  };
+  const Star stl = star(L, sdl, sdml, rosdl, itfl, isdml, sgl);
+  const Star str = star(Rt, sdr, sdmr, rosdr, itfr, isdmr, sgr);
+
+  // the double-star average (KH/mhd.cpp:337-360)
+  R sqrtrol, sqrtror, rsqrtrol, rsqrtror;
+  sq_rsq_pos(stl.ro, sqrtrol, rsqrtrol);
+  sq_rsq_pos(str.ro, sqrtror, rsqrtror);
+  const R abbx = fabs(bxs);
+  const R signbx = copysign1(bxs);
+  const R sign1 = copysign1(abbx - eps);
+  const R maxs1 = fmax(R(0), sign1);
+  const R mins1 = -fmin(R(0), sign1);
+  const R invsumro = dv(maxs1, sqrtrol + sqrtror);
+  const R tvy = invsumro * (sqrtrol * stl.vy + sqrtror * str.vy + signbx * (str.by - stl.by));
+  const R tvz = invsumro * (sqrtrol * stl.vz + sqrtror * str.vz + signbx * (str.bz - stl.bz));
+  const R tby = invsumro * (sqrtrol * str.by + sqrtror * stl.by + signbx * sqrtrol * sqrtror * (str.vy - stl.vy));
+  const R tbz = invsumro * (sqrtrol * str.bz + sqrtror * stl.bz + signbx * sqrtrol * sqrtror * (str.vz - stl.vz));
+
+  // the side whose term survives: right where sign(S_M) = -1 (KH/mhd.cpp:381-383)
+  const bool right = copysign1(sm) < R(0);
+  const R ro = pick(right, rol, ror), vx = pick(right, vxl, vxr);
+  const R vy = pick(right, L.vv, Rt.vv), vz = pick(right, L.vw, Rt.vw);
+  const R bn = pick(right, L.bn, Rt.bn), by = pick(right, L.bv, Rt.bv), bz = pick(right, L.bw, Rt.bw);
+  const R ee = pick(right, L.E, Rt.E), pt = pick(right, ptl, ptr);
+  const R X = pick(right, L.X, Rt.X), eb = pick(right, L.eb, Rt.eb);
+  const R sd = pick(right, sdl, sdr), isdm = pick(right, isdml, isdmr);
+  const R maxs = pick(right, stl.maxs, str.maxs), mins = pick(right, stl.mins, str.mins);
+  const R rost = pick(right, stl.ro, str.ro), vyst = pick(right, stl.vy, str.vy), vzst = pick(right, stl.vz, str.vz);
+  const R byst = pick(right, stl.by, str.by), bzst = pick(right, stl.bz, str.bz);
+  const R sqro = pick(right, sqrtrol, sqrtror), rsqro = pick(right, rsqrtrol, rsqrtror);
+  // the side's sign in the double-star energy and the Alfven speed: -1 left, +1 right
+  const R sgn = right ? R(1) : R(-1);
+
+  // its star state (KH/mhd.cpp:268-302 left, 304-333 right)
+  const R vxst = maxs * sm - mins * vx;
+  const R rxst = rost * vxst;
+  const R ryst = rost * vyst;
+  const R rzst = rost * vzst;
+  const R scst = rost * X;
+  const R vdbst = vxst * bxs + vyst * byst + vzst * bzst;
+  const R eest = maxs * (sd * ee - pt * vx + ptst * sm + bxs * (vx * bxs + vy * by + vz * bz - vdbst)) * isdm - mins * ee;
+
+  // its double-star state (KH/mhd.cpp:344-364)
+  const R vydst = vyst * mins1 + tvy;
+  const R rydst = ryst * mins1 + rost * tvy;
+  const R vzdst = vzst * mins1 + tvz;
+  const R rzdst = rzst * mins1 + rost * tvz;
+  const R bydst = byst * mins1 + tby;
+  const R bzdst = bzst * mins1 + tbz;
+  const R tdot = sm * bxs + vydst * bydst + vzdst * bzdst;
+  const R eedst = eest + sqro * (sgn * signbx) * (vdbst - tdot) * maxs1;
+
+  // its outer and Alfven wave speeds, clipped to the side (KH/mhd.cpp:367-375)
+  const R sast = (sm + (sgn * abbx) * rsqro) * maxs1;
+  const R ms = right ? fmax(sr, R(0)) : fmin(sl, R(0));
+  const R mst = right ? fmax(sast, R(0)) : fmin(sast, R(0));
+
+  // its physical flux (KH/mhd.cpp:1036-1076) and the flux of the side (KH/mhd.cpp:384-415)
+  const R rvx = ro * vx;
+  const R rvy = ro * vy;
+  const R rvz = ro * vz;
+  const R sc = ro * X;
+  nf.d = rvx + (rost - ro) * ms;
+  nf.u = (ro * vx * vx - bn * bn + pt) + (rxst - rvx) * ms;
+  nf.v = (ro * vy * vx - by * bn) + (ryst - rvy) * ms + (rydst - ryst) * mst;
+  nf.w = (ro * vz * vx - bz * bn) + (rzst - rvz) * ms + (rzdst - rzst) * mst;
+  nf.E = ((ee + pt) * vx - eb * bn) + (eest - ee) * ms + (eedst - eest) * mst;
+  nf.bv = (by * vx - vy * bn) + (byst - by) * ms + (bydst - byst) * mst;
+  nf.bw = (bz * vx - vz * bn) + (bzst - bz) * ms + (bzdst - bzst) * mst;
+  nf.sc = ro * X * vx + (scst - sc) * ms;
+}
+
 // Interface flux in Cartesian conserved order (F[0..9]) from reconstructed left/right
 // primitive states: HLLD for the MHD part plus the GLM (B_n, psi) flux
 // F(B_n) = (psiL+psiR)/2 - c_h (B_nR-B_nL)/2,  F(psi) = c_h^2 [(B_nL+B_nR)/2 - (psiR-psiL)/(2 c_h)]
@@ -396,13 +566,22 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
 // chg: the GLM speed c_h; hchg = 0.5/c_h (hoisted by the caller)
 template <int D, int EFLUX, typename R>
 __device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double chg, double hchg, double* F) {
-  RState<R> sl, sr;
   R rinvl, rinvr;
   recip2(ql[nden], qr[nden], rinvl, rinvr);
+  RFlux<R> nf;
+#if MHD_HLLD_NEAR
+  RPrim<R> sl, sr;
+  prim_to_rprim<D, EFLUX, R>(ql, rinvl, sl);
+  prim_to_rprim<D, EFLUX, R>(qr, rinvr, sr);
+  hlld_near<R>(sl, sr, nf);
+  const double bnl = double(sl.bn), bnr = double(sr.bn);
+#else
+  RState<R> sl, sr;
   prim_to_state<D, EFLUX, R>(ql, rinvl, sl);
   prim_to_state<D, EFLUX, R>(qr, rinvr, sr);
-  RFlux<R> nf;
   hlld<R>(sl, sr, nf);
+  const double bnl = double(sl.bu), bnr = double(sr.bu);
+#endif
   F[mden] = double(nf.d);
   F[meto] = double(nf.E);
   F[mrv1 + D] = double(nf.u);
@@ -412,7 +591,7 @@ __device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double ch
   F[mbm1 + (D + 2) % 3] = double(nf.bw);
   F[mst] = double(nf.sc);
   // GLM part in FP64 from the reconstructed (B_n, psi)
-  const double bnl = double(sl.bu), bnr = double(sr.bu), psl = double(sl.psi), psr = double(sr.psi);
+  const double psl = double(sl.psi), psr = double(sr.psi);
   F[mbm1 + D] = 0.5 * (psl + psr) - 0.5 * chg * (bnr - bnl);
   F[mbps] = (0.5 * (bnl + bnr) - hchg * (psr - psl)) * chg * chg;
 }
