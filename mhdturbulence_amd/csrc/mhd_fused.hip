@@ -1011,25 +1011,35 @@ bool fused_configure(FusedArgs& a, bool alone) {
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
-  // Chunk lengths.  A box run alone splits ny (nz) into the fewest chunks of at most 48 rows
-  // (64 planes in FP64, 16 in mixed precision) that still give the xy kernel as many
-  // wavefronts as the chip's SIMDs (256 CUs x 4) hold at once (one per SIMD in FP64, two in
-  // FP32) and the z kernel two per SIMD, as even as the split allows and no shorter than 8:
-  // fewer chunks re-read fewer halo rows (planes) and run fewer pre-passes.  The mixed z kernel is HBM-bound at two
-  // wavefronts per SIMD and gains more from wavefronts in flight than it loses to halo
-  // planes, hence its short cap.  In a batch (the other boxes fill the chip) 32 rows and 32
-  // planes.
-  auto split = [](int n, int cap, long long per_chunk, long long want) {
-    int c = (n + cap - 1) / cap;
-    while ((n + c - 1) / c > 8 && per_chunk * c < want) ++c;
-    return std::max(std::min(n, 8), (n + c - 1) / c);
+  // Chunk lengths.  Each chunk of rows (planes) is one wavefront's march, and the chip holds
+  // `slots` such wavefronts at once: one per SIMD (256 CUs x 4) in FP64, two in FP32, for
+  // both kernels.  A box run alone splits ny (nz) into c chunks of at most 48 rows (64
+  // planes) and no fewer than 8, with c minimising ceil(c * wavefronts per chunk / slots) *
+  // (chunk + 4): the rounds of wavefronts the launch takes times the rows each marches, its
+  // 4 halo rows (planes) included.  Longer chunks re-read fewer halo rows and run fewer
+  // pre-passes; a last round that fills a fraction of the chip costs a whole one.  In a
+  // batch (the other boxes fill the chip) 32 rows and 32 planes.
+  auto split = [](int n, int cap, long long per_chunk, long long slots) {
+    const int cmin = (n + cap - 1) / cap, cmax = std::max(cmin, (n + 7) / 8);
+    long long best_cost = -1;
+    int len = n;
+    for (int c = cmin; c <= cmax; ++c) {
+      const int l = (n + c - 1) / c;
+      if (l < 8 && c > cmin) break;
+      const long long cost = ((per_chunk * c + slots - 1) / slots) * (l + 4);
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        len = l;
+      }
+    }
+    return std::max(std::min(n, 8), len);
   };
+  const long long slots = a.mixed ? 2048 : 1024;
   const int ycap = std::min(alone ? 48 : 32, fused_max_ly(best, a.mixed));
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
-  a.ly = alone ? split(a.g.ny, ycap, strips, a.mixed ? 2048 : 1024) : std::min(ycap, a.g.ny);
+  a.ly = alone ? split(a.g.ny, ycap, strips, slots) : std::min(ycap, a.g.ny);
   const long long cols = (long long)a.ns * ((a.g.ny + 64 / best - 1) / (64 / best));
-  const int zcap = alone ? (a.mixed ? 16 : 64) : 32;
-  a.lz = alone ? split(a.g.nz, zcap, cols, 2048) : std::min(zcap, a.g.nz);
+  a.lz = alone ? split(a.g.nz, 64, cols, slots) : std::min(32, a.g.nz);
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets
 }
 

@@ -3,8 +3,8 @@
 * The benchmark's own box (bench.make_box): 256^3 KH shear layer, gamma 5/3, Bx 0.1,
   vy kick from seed 20261203, energy flux (v.B)B_n, CFL factor 0.05.  At 256^3 the fused
   kernels take tilings no smaller test reaches (FP64: 43-row y chunks of four-strip blocks
-  with the shared edge pre-pass and 64-plane z chunks; mixed: 16-plane z chunks and
-  block-cooperative FP32 strip edges).  The GPU's device-generated initial state is copied
+  with the shared edge pre-pass and 64-plane z chunks; mixed: 43-row y chunks of
+  block-cooperative FP32 strip edges and 64-plane z chunks).  The GPU's device-generated initial state is copied
   into the oracle, so both start from the same bits; then
   - FP64, fused kernels, 10 steps: rel L1 <= 1e-10, dt sequences equal to 1e-12;
   - the two-stage Runge-Kutta step at CFL 0.3, 5 steps: the same bar;
