@@ -1,8 +1,8 @@
 """Benchmark: cell-updates/s of the MHD time step on one MI355X, with its HBM roofline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256 | --shape NX NY NZ]
-                    [--mode 0|1] [--precision 0|1] [--stages 1|2 [--cfl C]] [--no-cpu]
-                    [--no-serve] [--no-rk2]
+                    [--mode 0|1] [--precision 0|1|2] [--stages 1|2 [--cfl C]] [--no-cpu]
+                    [--no-serve] [--no-rk2] [--no-dropin]
 
 Workload: a 256^3 FP64 Kelvin-Helmholtz shear-layer box (gamma 5/3, Bx = 0.1,
 vx = 0.5 tanh(y/0.05), seeded vy kick; periodic x,z, reflecting y), advanced with the
@@ -56,24 +56,25 @@ BYTES_PER_CELL = 2 * 10 * 8      # read + write the FP64 conserved state (NV = 1
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on MI355X, each
-# counter scaled by what it reads for a device copy of known size (fetch x2.0, write x1.0);
+# counter scaled by what it reads for a copy kernel of known size in the same run
+# (k_copy_interior, 1,342,177,280 B each way: fetch x2.0, write x1.0);
 # summaries and commands in profiles/fused_fp64_256cube_pmc.md and
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1323736.0 * 1024 * 2.0 + 1310723.0 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1373636.0 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
-    (256, 0, 2, "fused_xy"): 719814.0 * 1024 * 2.0 + 655363.0 * 1024 * 1.0,
-    (256, 0, 2, "fused_z_update"): 1280504.0 * 1024 * 2.0 + 1311744.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1324729.5 * 1024 * 2.0 + 1310722.6 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373824.1 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 2, "fused_xy"): 719849.3 * 1024 * 2.0 + 655360.6 * 1024 * 1.0,
+    (256, 0, 2, "fused_z_update"): 1112416.2 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
 }
 
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
 # MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
 MEASURED_VALU = {
-    (256, 0, 0, "fused_xy"): 390.3e6 * 64,
-    (256, 0, 0, "fused_z_update"): 240.1e6 * 64,
-    (256, 0, 2, "fused_xy"): 329.0e6 * 64,
-    (256, 0, 2, "fused_z_update"): 241.4e6 * 64,
+    (256, 0, 0, "fused_xy"): 369.5e6 * 64,
+    (256, 0, 0, "fused_z_update"): 227.1e6 * 64,
+    (256, 0, 2, "fused_xy"): 295.6e6 * 64,
+    (256, 0, 2, "fused_z_update"): 209.6e6 * 64,
 }
 # every SIMD issuing one wave64 VALU instruction per 4 clocks: 256 CUs x 4 SIMDs x 64 lanes
 # / 4 clocks x 2.4 GHz
