@@ -125,7 +125,12 @@ int mhd_update_primitive(mhd_ctx* ctx);
  * HBM.  *time is read as the starting time_sim and returns the final one; with
  * tmax > 0 the loop stops after the step that takes time past tmax (main.cpp:265).
  * dt_log (may be NULL, capacity nsteps) receives each dt.  *steps_done (may be NULL)
- * returns the steps applied.  P and U are both valid afterwards.
+ * returns the steps applied.  Every other entry point sees P and U of the final state.
+ * A fused run (mode 0) leaves that state in the fused loop's buffers, with P not yet
+ * recovered from it; the first call of another entry point that reads or writes P, U, F or
+ * the tiling brings P and U up to date, and a following fused run continues from the
+ * buffers and from the bounds the last step reduced (its first dt and c_h are not
+ * recomputed).  A run that met a bad cell brings them up to date before it returns.
  * mode: 0 = fused step kernel (default), 1 = staged kernels (one per reference function).
  * With mhd_config.stages = 2 each iteration is the two-stage Runge-Kutta step over that
  * loop body's update (one dt and c_h per iteration, from its starting state).
@@ -146,8 +151,8 @@ int mhd_run(mhd_ctx* ctx, int nsteps, double tmax, int mode, double* time, doubl
  * _step applies the rest of the iteration (SetBoundaryCondition ... UpdatePrimitvP) with
  * the fused kernels and that dt and c_h, and returns at once.  Between the calls the state
  * lives in the fused loop's buffers and P is not updated; the first call of any other entry
- * point that reads or writes P, U, F or the tiling first brings P and U up to date, as
- * mhd_run leaves them (so a stretch of steps ends there). */
+ * point that reads or writes P, U, F or the tiling first brings P and U up to date (so a
+ * stretch of steps ends there), as after a fused mhd_run. */
 int mhd_fused_begin_step(mhd_ctx* ctx, double* dt, double* chg, int* status);
 int mhd_fused_step(mhd_ctx* ctx);
 

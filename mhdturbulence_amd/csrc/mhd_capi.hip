@@ -85,8 +85,11 @@ struct mhd_ctx {
   double* snap_host = nullptr;
   hipEvent_t snap_ready = nullptr, snap_done = nullptr;
   bool snap_recorded = false, snap_pending = false;
-  // the fused loop driven one reference step at a time (mhd_fused_begin_step / _step) holds
-  // the state in U / U2 with P out of date until settle() brings them back
+  // the fused loop left live, by mhd_fused_begin_step / _step (the reference loop driven one
+  // step at a time) or by a fused mhd_run / mhd_profile_kernels: the state is in U or U2 by
+  // the parity of the iteration count, P is out of date and the ring holds the next step's
+  // dt / c_h.  The next fused run continues from there; settle() brings U and P back first
+  // for every other entry point.
   bool fz_live = false;
 };
 
@@ -170,8 +173,8 @@ int select_dev(mhd_ctx* c) {
 
 int read_scalars(mhd_ctx* c, StepScalars* h);
 
-// Ends a stretch of mhd_fused_step calls: the state is put back in U (from U2 after an odd
-// number of one-stage iterations) and P recovered from it, as mhd_run leaves them.
+// Ends a live stretch of the fused loop: the state is put back in U (from U2 after an odd
+// number of one-stage iterations, interior cells only) and P recovered from it.
 int settle(mhd_ctx* c) {
   if (!c->fz_live) return MHD_OK;
   c->fz_live = false;
@@ -184,8 +187,8 @@ int settle(mhd_ctx* c) {
   return MHD_OK;
 }
 
-// the context's device made current, and P / U up to date (every entry point but the
-// one-step fused pair goes through here)
+// the context's device made current, and P / U up to date (every entry point but the fused
+// runs, the one-step fused pair and the status read goes through here)
 int set_dev(mhd_ctx* c) {
   CHKRET(select_dev(c));
   return settle(c);
@@ -597,11 +600,29 @@ static int start_loop(mhd_ctx* c, int mode, double tmax, double time, int dt_cap
   return MHD_OK;
 }
 
-// loop state and buffers for nsteps iterations starting at *time
+// the loop scalars of a fused run that continues where the previous one left the state
+// live (fz_live): the caller's time and tmax and this run's counters; the iteration count,
+// whose parity says which buffer holds the state, and the ring slot the last step's
+// reduction filled (this run's first dt / c_h) are kept
+static int continue_loop(mhd_ctx* c, double tmax, double time, int dt_cap) {
+  double tt[2] = {time, tmax};
+  HIPCHK(hipMemcpyAsync(&c->sc->time, tt, sizeof tt, hipMemcpyHostToDevice, c->stream));
+  int ints[2] = {0, 0};  // steps_done, stopped
+  HIPCHK(hipMemcpyAsync(&c->sc->steps_done, ints, sizeof ints, hipMemcpyHostToDevice, c->stream));
+  const int cap_halt[2] = {dt_cap, 0};  // dt_log_cap, halt
+  HIPCHK(hipMemcpyAsync(&c->sc->dt_log_cap, cap_halt, sizeof cap_halt, hipMemcpyHostToDevice, c->stream));
+  return MHD_OK;
+}
+
+// loop state and buffers for nsteps iterations starting at *time.  A fused run continues a
+// fused run whose state is still live; anything else settles the state first.
 static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double time) {
-  CHKRET(need_grid(c));
+  if (!c) return fail(MHD_ERR_ARG, "null context");
+  if (!c->grid_set) return fail(MHD_ERR_STATE, "grid not set (mhd_set_grid or mhd_generate_grid first)");
   if (nsteps < 0) return fail(MHD_ERR_ARG, "nsteps must be >= 0");
   if (mode != 0 && mode != 1) return fail(MHD_ERR_ARG, "mode is 0 (fused) or 1 (staged)");
+  const bool cont = mode == 0 && c->fz_live;
+  CHKRET(cont ? select_dev(c) : set_dev(c));
   // device dt log, grown as needed (a captured graph holds its address)
   if (nsteps > c->dt_log_cap) {
     drop_graph(c);
@@ -612,23 +633,32 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
   if (mode == 0) CHKRET(ensure_fused(c));
   else CHKRET(ensure_flux(c));
   if (mode == 1 && two_stage(c)) CHKRET(dalloc(c, &c->U2, mhd::NCONS * c->g.vstride));
-  return start_loop(c, mode, tmax, time, c->dt_log_cap);
+  return cont ? continue_loop(c, tmax, time, c->dt_log_cap) : start_loop(c, mode, tmax, time, c->dt_log_cap);
 }
 
 // U2: the fused loop's second state copy; dev_log: the run's device dt log.  Waits for the
-// stream; allocates nothing.
+// stream; allocates nothing.  keep_live (fused runs of the context's own buffers): the state
+// stays where the last step left it, U or (after an odd number of one-stage iterations) U2,
+// with P out of date and the next step's dt / c_h in the ring, so a following fused run
+// continues without recomputing them; any other entry point settles it first (settle()).
+// A run that met a bad cell settles at once.
 static int finish_loop(mhd_ctx* c, int mode, const double* U2, const double* dev_log, int nsteps, double* time,
-                       double* dt_log, int* steps_done) {
+                       double* dt_log, int* steps_done, bool keep_live) {
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   if (mode == 0) {
-    // one-stage iterations leave the state in U2 after an odd number of them; its interior
-    // goes back to U (whose ghost cells the loop leaves as they were, as the reference's
-    // loop does) and P is recovered from it (two-stage iterations end in U)
-    if (!two_stage(c) && (h.iter & 1)) mhd::launch_copy_interior(c->g, U2, c->U, mhd::NCONS, c->stream);
-    mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (keep_live && !h.halt) {
+      c->fz_live = true;
+    } else {
+      // one-stage iterations leave the state in U2 after an odd number of them; its interior
+      // goes back to U (whose ghost cells the loop leaves as they were, as the reference's
+      // loop does) and P is recovered from it (two-stage iterations end in U)
+      if (!two_stage(c) && (h.iter & 1)) mhd::launch_copy_interior(c->g, U2, c->U, mhd::NCONS, c->stream);
+      mhd::launch_update_prim(c->g, c->cfg.gam, c->U, c->P, nullptr, c->stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(c->stream));
+      c->fz_live = false;
+    }
   }
   if (dt_log && h.steps_done > 0)
     HIPCHK(hipMemcpy(dt_log, dev_log, sizeof(double) * std::min(h.steps_done, nsteps), hipMemcpyDeviceToHost));
@@ -638,7 +668,7 @@ static int finish_loop(mhd_ctx* c, int mode, const double* U2, const double* dev
 }
 
 static int finish_run(mhd_ctx* c, int mode, int nsteps, double* time, double* dt_log, int* steps_done) {
-  return finish_loop(c, mode, c->U2, c->dt_log, nsteps, time, dt_log, steps_done);
+  return finish_loop(c, mode, c->U2, c->dt_log, nsteps, time, dt_log, steps_done, true);
 }
 
 int mhd_run(mhd_ctx* c, int nsteps, double tmax, int mode, double* time, double* dt_log, int* steps_done) {
@@ -766,7 +796,7 @@ int mhd_run_noalloc(mhd_ctx* c, int nsteps, double tmax, double* time, double* d
     CHKRET(start_loop(c, 0, tmax, t, NOALLOC_LOG));
     for (int s = 0; s < n; ++s) CHKRET(issue_step(c, 0, nullptr, a, dev_log));
     int got = 0;
-    CHKRET(finish_loop(c, 0, U2, dev_log, n, &t, dt_log ? dt_log + done : nullptr, &got));
+    CHKRET(finish_loop(c, 0, U2, dev_log, n, &t, dt_log ? dt_log + done : nullptr, &got, false));
     done += got;
     left -= n;
     if (got < n) break;  // stopped: past tmax, or a bad cell

@@ -347,6 +347,29 @@ def test_run_continues_across_calls(orc):
     assert np.all(rel_l1(o.interior(b.download_cons()), o.interior(o.U)) <= TOL_L1)
 
 
+@pytest.mark.parametrize("between", ["nothing", "download", "status", "tiling", "staged"])
+def test_fused_run_continues_from_live_state(orc, between):
+    """a fused run leaves the state live in its buffers (after 3 steps: in the second one);
+    the next fused run continues from there, or, after a call that settles the state, starts
+    again from P and U; either way 3 + 4 steps equal the oracle's 7"""
+    o, b = pair(orc, n=(24, 20, 16), problem=dict(bx0=0.1), gam=5.0 / 3.0, energy_flux=1)
+    t1, l1, _ = b.run(3, mode=0)
+    if between == "download":
+        b.download_prim()
+    elif between == "status":
+        assert b.status() == 0
+    elif between == "tiling":
+        b.set_tiling(*b.get_tiling())
+    t2, l2, _ = b.run(4, time=t1, mode=1 if between == "staged" else 0)
+    t_o, log_o = o.run(7)
+    np.testing.assert_allclose(np.concatenate([l1, l2]), log_o, rtol=1e-12)
+    assert t2 == pytest.approx(t_o, rel=1e-12)
+    errP = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+    errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+    say(f"between {between}: max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+    assert np.all(errP <= TOL_L1) and np.all(errU <= TOL_L1)
+
+
 @pytest.mark.parametrize("mode,steps", [(0, 3), (0, 4), (1, 3)])
 def test_loop_leaves_ghost_cells_of_u(orc, mode, steps):
     """The reference's loop writes only interior cells of U (src_cpp_kokkos_host/mhd.cpp:1431,
