@@ -102,3 +102,22 @@ def test_tprof_tool_small(tmp_path):
     row = [float(x) for x in lines[2].split()]
     assert row[0] == 0.0 and row[3] == pytest.approx(1.2e-3) and 0.0 < row[1] and 0.0 < row[2] < 1e-2
     assert np.isfinite(r["growth_rate"])
+
+
+def test_tprof_slice_in_vissimple_format(tmp_path):
+    """--slice: the first interior z-plane of the final state in the reference sample's
+    columns (x y d vx vy p phi X1), one row per cell, readable by the statistics helper"""
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from vissimple_stats import read_slices, slice_stats
+    from mhdturbulence_amd import tprof
+    path = tmp_path / "slice.csv"
+    r = tprof.run(20, 0.05, 0.05, None, fit=(0.0, 0.05), gam=5.0 / 3.0, grid_shift=0, slice_path=str(path))
+    assert r["status"] == 0 and r["gamma"] == pytest.approx(5.0 / 3.0)
+    a = read_slices([path])
+    assert a.shape == (20 * 20, 8)
+    # x runs fastest over the unshifted centres of [-0.5, 0.5], y over [-1, 1]
+    assert a[0, 0] == pytest.approx(-0.475, abs=1e-3) and a[1, 0] == pytest.approx(-0.425, abs=1e-3)
+    assert a[0, 1] == pytest.approx(-0.95, abs=1e-3) and a[20, 1] == pytest.approx(-0.85, abs=1e-3)
+    s = slice_stats(a)
+    assert 0.9 < s["d"]["mean"] < 1.1 and 0.0 < s["mix"] < 0.25

@@ -575,3 +575,19 @@ def test_splitmix64_known_values(orc):
     # published first outputs of splitmix64 seeded with 0 (state increments before mixing)
     assert orc.splitmix64(0) == 0xE220A8397B1DCDAF
     assert orc.splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+
+
+def test_vissimple_t15_fixture_statistics():
+    """the committed statistics of the reference's t = 15 sample (make_vissimple_t15_stats.py)
+    hold the loose ranges read off those files: rho in [0.89, 1.09], p in [0.98, 1.28] (three
+    digits), 22,500 cells of a 150^3 run's first interior z-plane, a well-mixed scalar"""
+    import json
+    with open(os.path.join(HERE, "golden", "vissimple_t15_stats.json")) as f:
+        s = json.load(f)
+    assert s["cells"] == 150 * 150
+    assert s["d"]["min"] == pytest.approx(0.887) and s["d"]["max"] == pytest.approx(1.09)
+    assert s["p"]["min"] == pytest.approx(0.978) and s["p"]["max"] == pytest.approx(1.28)
+    for name in ("d", "vx", "vy", "p", "X1"):
+        q = [s[name][k] for k in ("min", "q5", "q25", "q50", "q75", "q95", "max")]
+        assert q == sorted(q) and s[name]["std"] > 0.0
+    assert 0.2 < s["mix"] <= 0.25 and 0.0 < s["vy_rms"] < 0.5
