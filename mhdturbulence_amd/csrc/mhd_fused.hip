@@ -1003,11 +1003,19 @@ size_t fused_xy_lds(const FusedArgs& a) {
 }
 
 bool fused_configure(FusedArgs& a, bool alone) {
-  // strip width with the fewest idle lanes (ties: the wider strip)
+  // strip width with the fewest idle lanes (ties: the wider strip).  In FP32, 64-cell strips
+  // whenever at most a quarter of their lanes idle: there the narrower strips lose more to
+  // their y chunks, which the pre-pass slots of 2 or 4 strips per wavefront cap at 32 or 16
+  // rows, and to their edge faces than the wide strip loses to idle lanes (48- and 96-wide
+  // boxes: 22 % and 5 % faster per cell in a batch of precision 1)
   int best = 64, best_lanes = 1 << 30;
   for (int sw : {64, 32, 16}) {
     const int lanes = (a.g.nx + sw - 1) / sw * sw;
     if (lanes < best_lanes) { best = sw; best_lanes = lanes; }
+  }
+  if (a.mixed) {
+    const int lanes64 = (a.g.nx + 63) / 64 * 64;
+    if (4 * (lanes64 - a.g.nx) <= lanes64) best = 64;
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
