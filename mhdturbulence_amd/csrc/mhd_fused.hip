@@ -303,7 +303,9 @@ __host__ __device__ inline size_t xy_wave_lds(int sw, int ly) { return (size_t)(
 // pre-pass they save.
 template <typename R, bool WIDE>
 __host__ __device__ inline bool xy_coop(const FusedArgs& a) {
-  if (!(WIDE && MHD_XY_COOP && sizeof(R) == 4)) return false;
+  // MHD_XY_COOP=2: FP64 strips too (a measurement switch; the default keeps FP64 on the
+  // pre-pass, see above)
+  if (!(WIDE && MHD_XY_COOP && (sizeof(R) == 4 || MHD_XY_COOP == 2))) return false;
   if (a.ns % WAVES == 0) return true;
   // one strip per x-row shares an edge only with itself, across a periodic x
   if (a.ns == 1 && !(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return false;
@@ -320,13 +322,15 @@ __host__ __device__ inline bool xy_coop(const FusedArgs& a) {
 // face of the last strip).
 template <typename R, bool WIDE>
 __host__ __device__ inline bool xy_share(const FusedArgs& a) {
-  return WIDE && MHD_XY_SHARE && sizeof(R) == 8 && a.ns % WAVES == 0;
+  return WIDE && MHD_XY_SHARE && MHD_XY_COOP != 2 && sizeof(R) == 8 && a.ns % WAVES == 0;
 }
 static_assert((WAVES + 1) * (NCONS + 2 * NPRIM) * sizeof(double) <= WAVES * 2 * (NCONS + NPRIM) * sizeof(double),
               "the shared pre-pass slots fit the block's per-wave slots");
 // static LDS of the edge exchange (FP32 only)
 template <typename R>
-__host__ __device__ constexpr int xy_edge_len(int per_wave) { return sizeof(R) == 4 ? WAVES * per_wave : 1; }
+__host__ __device__ constexpr int xy_edge_len(int per_wave) {
+  return (sizeof(R) == 4 || MHD_XY_COOP == 2) ? WAVES * per_wave : 1;
+}
 //
 // FP64 (MHD_XY_RING=1): the window rows and the previous pass's upper y state live in this
 // wave's LDS ring, ring[(slot * NPRIM + n) * 64 + lane]: slots 0..2 hold rows j-1, j, j+1 in
