@@ -641,13 +641,14 @@ static int prepare_run(mhd_ctx* c, int nsteps, double tmax, int mode, double tim
 // stays where the last step left it, U or (after an odd number of one-stage iterations) U2,
 // with P out of date and the next step's dt / c_h in the ring, so a following fused run
 // continues without recomputing them; any other entry point settles it first (settle()).
-// A run that met a bad cell settles at once.
+// A run that stopped (past tmax, or at a bad cell) settles at once: its stop flag would
+// otherwise hold the one-step fused pair, which continues a live state as it finds it.
 static int finish_loop(mhd_ctx* c, int mode, const double* U2, const double* dev_log, int nsteps, double* time,
                        double* dt_log, int* steps_done, bool keep_live) {
   StepScalars h;
   CHKRET(read_scalars(c, &h));
   if (mode == 0) {
-    if (keep_live && !h.halt) {
+    if (keep_live && !h.halt && !h.stopped) {
       c->fz_live = true;
     } else {
       // one-stage iterations leave the state in U2 after an odd number of them; its interior

@@ -370,6 +370,31 @@ def test_fused_run_continues_from_live_state(orc, between):
     assert np.all(errP <= TOL_L1) and np.all(errU <= TOL_L1)
 
 
+@pytest.mark.parametrize("stop", [False, True])
+def test_fused_step_pair_after_run(orc, stop):
+    """the one-step fused pair (the C++ drop-in's recorded step) after a fused run: one that
+    ran all its steps (its state left live, 3 steps: in the second buffer) or one that
+    stopped past tmax; two more steps through the pair equal the oracle's continuation"""
+    import ctypes as C
+    o, b = pair(orc, n=(24, 20, 16), problem=dict(bx0=0.1), gam=5.0 / 3.0, energy_flux=1)
+    tmax = 0.0
+    if stop:
+        tmax = 2.5 * o.control_timestep()
+    t, log, done = b.run(50 if stop else 3, tmax=tmax, mode=0)
+    assert done == 3
+    dts = list(log)
+    for _ in range(2):
+        dt, ch, st = C.c_double(), C.c_double(), C.c_int()
+        assert b.L.mhd_fused_begin_step(b.h, C.byref(dt), C.byref(ch), C.byref(st)) == 0 and st.value == 0
+        assert b.L.mhd_fused_step(b.h) == 0
+        dts.append(dt.value)
+    _, log_o = o.run(5)
+    np.testing.assert_allclose(dts, log_o, rtol=1e-12)
+    errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+    say(f"stop {stop}: max relL1 U={errU.max():.3e}")
+    assert np.all(errU <= TOL_L1)
+
+
 @pytest.mark.parametrize("mode,steps", [(0, 3), (0, 4), (1, 3)])
 def test_loop_leaves_ghost_cells_of_u(orc, mode, steps):
     """The reference's loop writes only interior cells of U (src_cpp_kokkos_host/mhd.cpp:1431,
