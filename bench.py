@@ -56,14 +56,14 @@ BYTES_PER_CELL = 2 * 10 * 8      # read + write the FP64 conserved state (NV = 1
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on MI355X, each
-# counter scaled by what it reads for a copy kernel of known size in the same run
+# counter scaled by what it reads for a copy kernel of known size on the same build
 # (k_copy_interior, 1,342,177,280 B each way: fetch x2.0, write x1.0);
 # summaries and commands in profiles/fused_fp64_256cube_pmc.md and
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1324729.5 * 1024 * 2.0 + 1310722.6 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1373824.1 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 1319565.1 * 1024 * 2.0 + 1310723.1 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1374082.9 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
     (256, 0, 2, "fused_xy"): 719849.3 * 1024 * 2.0 + 655360.6 * 1024 * 1.0,
     (256, 0, 2, "fused_z_update"): 1112416.2 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
 }
@@ -71,7 +71,7 @@ MEASURED_TRAFFIC = {
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
 # MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
 MEASURED_VALU = {
-    (256, 0, 0, "fused_xy"): 369.5e6 * 64,
+    (256, 0, 0, "fused_xy"): 362.3e6 * 64,
     (256, 0, 0, "fused_z_update"): 227.1e6 * 64,
     (256, 0, 2, "fused_xy"): 295.6e6 * 64,
     (256, 0, 2, "fused_z_update"): 209.6e6 * 64,
