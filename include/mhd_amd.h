@@ -237,7 +237,9 @@ int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_
 /* tile shape of a single box's fused kernels: sw 16/32/64 (x-strip width), 1 <= ly <= ny
  * (y chunk; also at most what the xy kernel's LDS for strip-edge faces allows per block:
  * FP64 46, 23, 11 rows (160 KiB less its window rings) and mixed 64, 32, 16 rows (64 KiB)
- * for sw 64, 32, 16), 1 <= lz <= min(nz, 64) (z chunk) */
+ * for sw 64, 32, 16; 64 rows where the FP64 march runs two blocks per CU with its strip
+ * edges shared in the block: sw 64, nx 64, 128 or 256, x periodic), 1 <= lz <= min(nz, 64)
+ * (z chunk) */
 int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
 int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
 

@@ -121,6 +121,7 @@ int fused_args(const mhd_ctx* c, double* U2, double* acc, mhd::FusedArgs& a) {
   a.g = c->g;
   a.mixed = c->cfg.precision != 0;
   a.acc32 = c->cfg.precision == 2;
+  for (int q = 0; q < 6; ++q) a.bc[q] = c->cfg.bc[q];
   if (!mhd::fused_configure(a))
     return fail(MHD_ERR_UNSUPPORTED, "fused step: a variable column of this box spans 4 GiB or more "
                                      "(more than 2^29 cells); use the staged step (mode 1)");
@@ -853,14 +854,14 @@ int mhd_set_tiling(mhd_ctx* c, int sw, int ly, int lz) {
   if ((sw != 16 && sw != 32 && sw != 64) || ly < 1 || lz < 1 || ly > c->g.ny || lz > c->g.nz ||
       lz > 64)
     return fail(MHD_ERR_ARG, "tiling: sw in {16,32,64}, 1 <= ly <= ny, 1 <= lz <= min(nz, 64)");
-  if (ly > mhd::fused_max_ly(sw, c->cfg.precision))
-    return fail(MHD_ERR_ARG, "tiling: ly above " + std::to_string(mhd::fused_max_ly(sw, c->cfg.precision)) +
-                                 " (LDS of the xy kernel) for this strip width and precision");
   CHKRET(set_dev(c));
   CHKRET(ensure_fused(c));
   mhd::FusedArgs a = c->fa;
   a.sw = sw;
   a.ns = (c->g.nx + sw - 1) / sw;
+  if (ly > mhd::fused_ly_cap(a))
+    return fail(MHD_ERR_ARG, "tiling: ly above " + std::to_string(mhd::fused_ly_cap(a)) +
+                                 " (LDS of the xy kernel) for this strip width and precision");
   a.ly = ly;
   a.lz = lz;
   c->fa = a;

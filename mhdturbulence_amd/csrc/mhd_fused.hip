@@ -330,6 +330,24 @@ template <typename R, bool WIDE>
 __host__ __device__ inline bool xy_share(const FusedArgs& a) {
   return WIDE && MHD_XY_SHARE && MHD_XY_COOP != 2 && sizeof(R) == 8 && a.ns % WAVES == 0;
 }
+
+// Two FP64 blocks per CU (MHD_XY_DUO=1): where every strip edge of a block has its
+// neighbour in the block (full-wave strips, x periodic, a block holding one or more whole
+// x-rows: nx = 64, 128 or 256), the FP64 march takes the block-shared edges and no
+// pre-pass, and its window ring drops the row that the carried registers already hold
+// (3 slots: rows j, j+1 and the upper state of row j-1; row j-1 is the carry).  A block
+// then takes 77.5 KB of LDS, so two blocks share a CU: two wavefronts per SIMD, each
+// covering the other's barrier, LDS and memory waits.
+#ifndef MHD_XY_DUO
+#define MHD_XY_DUO 1
+#endif
+constexpr int WIN_RING3 = 3 * NPRIM * 64;
+__host__ __device__ inline bool xy_duo(const FusedArgs& a) {
+  if (!(MHD_XY_DUO && MHD_XY_RING && MHD_XY_CARRY && !a.mixed && a.sw == 64 && a.g.nx % 64 == 0)) return false;
+  if (!(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return false;
+  // the block's waves: the four strips of one x-row, or 4/ns whole x-rows of one y chunk
+  return a.ns == WAVES || ((a.ns == 1 || a.ns == 2) && a.g.nz % (WAVES / a.ns) == 0);
+}
 static_assert((WAVES + 1) * (NCONS + 2 * NPRIM) * sizeof(double) <= WAVES * 2 * (NCONS + NPRIM) * sizeof(double),
               "the shared pre-pass slots fit the block's per-wave slots");
 // static LDS of the edge exchange (FP32 only)
@@ -345,9 +363,12 @@ __host__ __device__ constexpr int xy_edge_len(int per_wave) {
 // registers to the Riemann solves.  FP32 keeps the window in registers.
 // ACC32: the divergence is stored in FP32 (FusedArgs::acc32), fixed at compile time so the
 // other kernels carry nothing of it
-template <int EFLUX, typename R, bool WIDE, bool ACC32 = false>
+// DUO: the two-blocks-per-CU FP64 march (xy_duo holds): block-shared edges on both sides of
+// every strip, 3-slot ring
+template <int EFLUX, typename R, bool WIDE, bool ACC32 = false, bool DUO = false>
 __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, int lane, char* lds, R* ring, R* edgeP,
                                         double* edgeF, int wv) {
+  static_assert(!DUO || (WIDE && sizeof(R) == 8 && MHD_XY_RING && MHD_XY_CARRY), "DUO: FP64 full-wave strips");
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -391,18 +412,18 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   // block-uniform: a block holds 4 neighbouring strips of one x-row (ns a multiple of 4), or
   // 4/ns whole x-rows of the same y chunk (ns = 1 or 2 and nz a multiple of 4/ns), so all of
   // its waves march the same rows and take the same barriers
-  const bool coop = xy_coop<R, WIDE>(a);
+  const bool coop = DUO || xy_coop<R, WIDE>(a);
   const int rw = min(a.ns, WAVES);  // strips of one x-row in the block
   const int pos = wv % rw, grp = wv - pos;
   const bool wrap = coop && a.ns <= WAVES && a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC;
-  const bool coop_l = coop && (pos > 0 || wrap);
-  const bool coop_r = coop && (pos < rw - 1 || wrap);
+  const bool coop_l = DUO || (coop && (pos > 0 || wrap));
+  const bool coop_r = DUO || (coop && (pos < rw - 1 || wrap));
   const int nb_l = grp + (pos + rw - 1) % rw, nb_r = grp + (pos + 1) % rw;
   // the edge cell (side 0: first, 1: last) of row jr that wavefront `wave` published
   auto edge_cell = [&](int wave, int jr, int side) { return edgeP + wave * XY_EDGE_R + ((jr & 3) * 2 + side) * NPRIM; };
   R* const edge_rt = edgeP + wv * XY_EDGE_R + 8 * NPRIM;
 
-  const bool share = xy_share<R, WIDE>(a);
+  const bool share = !DUO && xy_share<R, WIDE>(a);
   // the pre-pass slots this wave's march reads, per row r: left / right face flux and the
   // outside neighbour cells
   const double* fL;
@@ -458,7 +479,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   const int nsides = share ? 0 : 2 - (coop_l ? 1 : 0) - (coop_r ? 1 : 0);
   const int nf = SPW * nsides * nrows;
   for (int f = lane; f < nf; f += 64) {
-    const int r = f % nrows, side = side0 + (f / nrows) % nsides, sb = f / (nsides * nrows);
+    const int r = f % nrows, side = side0 + (f / nrows) % max(nsides, 1), sb = f / (max(nsides, 1) * nrows);
     const int kr = min(NGH + kg * SPW + sb, NGH + g.nz - 1);
     const int fc = side ? ilast + 1 : i0;  // the face between cells fc-1 and fc
     R q[4][NPRIM];
@@ -508,13 +529,21 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   for (int n = 0; n < NPRIM; ++n) upprev[n] = R(0);
 #pragma unroll
   for (int m = 0; m < NCONS; ++m) Fyprev[m] = 0.0;
-  if constexpr (PRING) {
+  // DUO: slots 0 and 1 hold rows j and j+1 in rotation (s0, sp), slot 2 the upper state of
+  // row j-1; row j-1 itself is the carry Px
+  if constexpr (DUO) {
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) Px[n] = Pm[n];
+    wr(0, P0);
+    wr(1, Pp);
+    wr(2, upprev);
+  } else if constexpr (PRING) {
     wr(0, Pm);
     wr(1, P0);
     wr(2, Pp);
     wr(3, upprev);
   }
-  int sm = 0, s0 = 1, sp = 2;
+  int sm = 0, s0 = DUO ? 0 : 1, sp = DUO ? 1 : 2;
 
   // Pass j: the window holds rows j-1, j, j+1.  The raw state of row j+2 is loaded at the
   // start and turned into primitives after both Riemann solves have covered the load
@@ -606,7 +635,15 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     const double yinv = lane_row(yinv_l, max(row - j0, 0));
     // y states of row j and the y face j-1/2
     R up[NPRIM], dn[NPRIM];
-    if constexpr (PRING) {
+    if constexpr (DUO) {
+      rdr(s0, P0);
+      rdr(sp, Pp);
+      cell_states<R>(Px, P0, Pp, up, dn);
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) Px[n] = P0[n];
+      rdr(2, upprev);
+      wr(2, up);
+    } else if constexpr (PRING) {
       rdr(sm, Pm);
       rdr(s0, P0);
       rdr(sp, Pp);
@@ -627,7 +664,8 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     R Pn[NPRIM];
     un[mrv2] = negate_if(un[mrv2], neg_n);
     recover_in_place<R>(un, a.gam, Pn);
-    if constexpr (PRING) wr(sm, Pn);
+    if constexpr (DUO) wr(s0, Pn);  // row j's slot: row j is the carry now
+    else if constexpr (PRING) wr(sm, Pn);
     if (coop && (lane == 0 || lane == 63)) {
       R* e = edge_cell(wv, j + 2, lane == 63);
 #pragma unroll
@@ -640,7 +678,11 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
       if constexpr (ACC32) store_acc32(reinterpret_cast<float*>(a.acc), vs, g.cell(k, row, i), d);
       else store_acc(a.acc, g.cell(k, row, i), d);
     }
-    if constexpr (PRING) {
+    if constexpr (DUO) {
+      const int t0 = s0;
+      s0 = sp;
+      sp = t0;
+    } else if constexpr (PRING) {
       const int t0 = sm;
       sm = s0;
       s0 = sp;
@@ -677,6 +719,18 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
   xy_body<EFLUX, R, WIDE, ACC32>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
                           smem + wv * xy_wave_lds<R>(a.sw, a.ly), xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0),
                           xedgeP, xedgeF, wv);
+}
+
+// FP64 march at two blocks per CU (xy_duo): static LDS only, 4 x 18 KiB of window rings and
+// the edge exchange, 77.5 KB per block
+template <int EFLUX>
+__global__ __launch_bounds__(256, 2) void k_fused_xy_duo(FusedArgs a) {
+  __shared__ double xring[WAVES * WIN_RING3];
+  __shared__ double xedgeP[WAVES * XY_EDGE_R];
+  __shared__ double xedgeF[WAVES * NCONS];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
+  xy_body<EFLUX, double, true, false, true>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63, nullptr,
+                                            xring + wv * WIN_RING3, xedgeP, xedgeF, wv);
 }
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
@@ -953,6 +1007,12 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 
 template <int EFLUX, typename R, bool ACC32>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
+  if constexpr (sizeof(R) == 8) {
+    if (a.duo && xy_duo(a)) {
+      hipLaunchKernelGGL((k_fused_xy_duo<EFLUX>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), 0, s, a);
+      return;
+    }
+  }
   if (xy_wide(a))
     hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true, ACC32>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
                        fused_xy_lds(a), s, a);
@@ -1014,12 +1074,18 @@ size_t fused_xy_lds_max(int mixed) {
   return std::min(FUSED_XY_LDS_MAX, XY_LDS_PER_CU - fixed);
 }
 
+int fused_ly_cap(const FusedArgs& a) {
+  if (a.duo && xy_duo(a)) return 64;  // one row constant per lane, no pre-pass slots
+  return std::min(64, fused_max_ly(a.sw, a.mixed));
+}
+
 int fused_max_ly(int sw, int mixed) {
   const size_t per_row = (size_t)WAVES * (mixed ? xy_wave_lds<float>(sw, 1) : xy_wave_lds<double>(sw, 1));
   return (int)(fused_xy_lds_max(mixed) / per_row);
 }
 
 size_t fused_xy_lds(const FusedArgs& a) {
+  if (a.duo && xy_duo(a)) return 0;
   return (size_t)WAVES * (a.mixed ? xy_wave_lds<float>(a.sw, a.ly) : xy_wave_lds<double>(a.sw, a.ly));
 }
 
@@ -1064,9 +1130,21 @@ bool fused_configure(FusedArgs& a, bool alone) {
     return std::max(std::min(n, 8), len);
   };
   const long long slots = a.mixed ? 2048 : 1024;
-  const int ycap = std::min(alone ? 48 : 32, fused_max_ly(best, a.mixed));
+  // the two-blocks-per-CU FP64 march (xy_duo) holds two wavefronts per SIMD and has no
+  // pre-pass slots to cap its chunk (64 rows: one per lane of its row constants)
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
-  a.ly = alone ? split(a.g.ny, ycap, strips, slots) : std::min(ycap, a.g.ny);
+  bool duo = alone && xy_duo(a);
+  if (duo) {
+    // only where its chunks give more blocks than the chip has CUs: a smaller launch
+    // leaves CUs idle either way and pays the block barriers for nothing (64^3: 7 % slower)
+    a.ly = split(a.g.ny, 64, strips, 2048);
+    duo = strips * ((a.g.ny + a.ly - 1) / a.ly) > (long long)WAVES * 256;
+  }
+  a.duo = duo ? 1 : 0;
+  if (!duo) {
+    const int ycap = std::min(alone ? 48 : 32, fused_max_ly(best, a.mixed));
+    a.ly = alone ? split(a.g.ny, ycap, strips, slots) : std::min(ycap, a.g.ny);
+  }
   const long long cols = (long long)a.ns * ((a.g.ny + 64 / best - 1) / (64 / best));
   a.lz = alone ? split(a.g.nz, 64, cols, slots) : std::min(32, a.g.nz);
   return a.g.vstride * sizeof(double) < (size_t(1) << 32);  // 32-bit column byte offsets

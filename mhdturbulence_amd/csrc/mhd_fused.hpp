@@ -42,10 +42,13 @@ struct FusedArgs {
   // U1 -> U0 averaged with the U0 it replaces; the state is in U0 after every iteration, and
   // only the second stage reduces the next iteration's dt and c_h
   int stage;
+  // 1: the FP64 xy march runs two blocks per CU (set by fused_configure for a box run alone
+  // whose strips and boundaries allow it; batches keep 0)
+  int duo;
 };
 
-// strip width and chunk lengths for a geometry (a.g and a.mixed set), for a box run alone or
-// in a batch; false if a variable column of the box spans 4 GiB or more (the fused kernels
+// strip width and chunk lengths for a geometry (a.g, a.mixed and a.bc set), for a box run
+// alone or in a batch; false if a variable column of the box spans 4 GiB or more (the fused kernels
 // address columns with 32-bit offsets)
 bool fused_configure(FusedArgs& a, bool alone = true);
 
@@ -58,6 +61,10 @@ constexpr size_t XY_LDS_PER_CU = 160 * 1024;
 size_t fused_xy_lds(const FusedArgs& a);
 size_t fused_xy_lds_max(int mixed);
 int fused_max_ly(int sw, int mixed);
+// the longest y chunk the single-box xy launch takes for a box's strip width (a.sw, a.ns,
+// a.duo set): fused_max_ly, or 64 for the two-blocks-per-CU FP64 march, which has no
+// pre-pass slots
+int fused_ly_cap(const FusedArgs& a);
 
 // blocks of 256 threads each kernel needs for one box
 int fused_blocks_xy(const FusedArgs& a);
