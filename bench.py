@@ -22,7 +22,8 @@ variables = 160 B) per launch of the dominant kernel / that kernel's average lau
 duration in the timed region, against 8 TB/s.
 valu: the same launch's VALU lane-instructions (rocprofv3 SQ_INSTS_VALU x 64, measured at
 256^3) over its duration, against every SIMD issuing a wave64 instruction each 4 clocks at
-2.4 GHz; the FP64 kernels run one wavefront per SIMD and are bound by this issue.
+2.4 GHz; the FP64 xy kernel is bound by this issue (two wavefronts per SIMD where a block
+holds whole periodic x-rows, one elsewhere).
 host_transfer: the reference's host-array crossings timed once (upload of P and U, download
 of P, between pageable numpy arrays allocated once), and the rate if every step made that
 round trip;
@@ -62,8 +63,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 1319565.1 * 1024 * 2.0 + 1310723.1 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1374082.9 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 694800.7 * 1024 * 2.0 + 1310760.5 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373941.3 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
     (256, 0, 2, "fused_xy"): 719849.3 * 1024 * 2.0 + 655360.6 * 1024 * 1.0,
     (256, 0, 2, "fused_z_update"): 1112416.2 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
 }
@@ -71,7 +72,7 @@ MEASURED_TRAFFIC = {
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
 # MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
 MEASURED_VALU = {
-    (256, 0, 0, "fused_xy"): 362.3e6 * 64,
+    (256, 0, 0, "fused_xy"): 354.7e6 * 64,
     (256, 0, 0, "fused_z_update"): 227.1e6 * 64,
     (256, 0, 2, "fused_xy"): 295.6e6 * 64,
     (256, 0, 2, "fused_z_update"): 209.6e6 * 64,

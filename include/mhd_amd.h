@@ -243,6 +243,17 @@ int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_
 int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
 int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
 
+/* FP64 xy kernel of a single box's fused step (no counterpart in the reference: a tuning
+ * switch beside mhd_set_tiling).  n = 2: two blocks per CU with the strip edges shared in the
+ * block, which needs sw 64, nx 64, 128 or 256, x periodic and, for nx 64 / 128, nz a multiple
+ * of 4 / 2 (else MHD_ERR_ARG); n = 1: one block per CU with the strip-edge pre-pass, which
+ * needs ly within that kernel's LDS limit (mhd_set_tiling); n = 0: the default rule (two
+ * blocks where the box allows it and that launch has more blocks than the GPU has CUs).  The
+ * results agree to round-off (the strip-edge faces are computed by different code).  Mixed
+ * precision ignores it.  mhd_get_xy_blocks_per_cu gives the kernel the next step takes. */
+int mhd_set_xy_blocks_per_cu(mhd_ctx* ctx, int n);
+int mhd_get_xy_blocks_per_cu(mhd_ctx* ctx, int* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1074,6 +1074,13 @@ size_t fused_xy_lds_max(int mixed) {
   return std::min(FUSED_XY_LDS_MAX, XY_LDS_PER_CU - fixed);
 }
 
+bool fused_xy_two_blocks_ok(const FusedArgs& a) {
+  FusedArgs d = a;
+  d.mixed = 0;
+  return xy_duo(d);
+}
+bool fused_xy_two_blocks(const FusedArgs& a) { return a.duo && xy_duo(a); }
+
 int fused_ly_cap(const FusedArgs& a) {
   if (a.duo && xy_duo(a)) return 64;  // one row constant per lane, no pre-pass slots
   return std::min(64, fused_max_ly(a.sw, a.mixed));

@@ -65,6 +65,10 @@ int fused_max_ly(int sw, int mixed);
 // a.duo set): fused_max_ly, or 64 for the two-blocks-per-CU FP64 march, which has no
 // pre-pass slots
 int fused_ly_cap(const FusedArgs& a);
+// whether the FP64 xy march can run two blocks per CU at a box's strip width and boundaries,
+// and whether the next single-box launch does (a.duo set and allowed)
+bool fused_xy_two_blocks_ok(const FusedArgs& a);
+bool fused_xy_two_blocks(const FusedArgs& a);
 
 // blocks of 256 threads each kernel needs for one box
 int fused_blocks_xy(const FusedArgs& a);

@@ -221,6 +221,16 @@ class Box:
         """x-strip width (16/32/64) and y / z march chunk lengths of the fused kernels"""
         capi.check(self.L.mhd_set_tiling(self.h, sw, ly, lz))
 
+    def set_xy_blocks_per_cu(self, n: int):
+        """FP64 xy kernel: 2 = two blocks per CU with block-shared strip edges (where the box
+        allows it), 1 = one block per CU with the strip-edge pre-pass, 0 = the default rule"""
+        capi.check(self.L.mhd_set_xy_blocks_per_cu(self.h, n))
+
+    def get_xy_blocks_per_cu(self) -> int:
+        n = C.c_int()
+        capi.check(self.L.mhd_get_xy_blocks_per_cu(self.h, C.byref(n)))
+        return n.value
+
     def get_tiling(self) -> tuple[int, int, int]:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         capi.check(self.L.mhd_get_tiling(self.h, C.byref(a), C.byref(b), C.byref(c)))

@@ -327,6 +327,68 @@ def test_block_cooperative_strip_edges(orc, shape, bc):
     assert rel_l1(mixed[0], mixed[1]).max() < 1e-8
 
 
+@pytest.mark.parametrize("shape,steps,tiling", [
+    ((256, 10, 8), 12, None),       # four strips of one x-row per block
+    ((128, 8, 10), 12, None),       # two strips per row, two rows per block
+    ((64, 6, 8), 12, None),         # one strip per row, four rows per block
+    ((64, 40, 24), 40, (64, 16, 24)),   # three y chunks: halo rows between chunks
+    ((128, 70, 12), 20, (64, 64, 12)),  # 64-row chunks, the longest, and a 6-row last one
+])
+def test_fp64_two_blocks_per_cu(orc, shape, steps, tiling):
+    """The FP64 xy march at two blocks per CU (strip edges shared in the block, 3-slot window
+    ring with row j-1 taken from the carried registers), forced on boxes below the default
+    rule's block count, against the oracle; then the one-block kernel on the same box"""
+    kw = dict(bc=KH_BC, energy_flux=1, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), grid_shift=0)
+    ic = dict(bx0=0.1, a=0.06, amp=0.04)
+    for n in (2, 1):
+        say("two blocks per CU" if n == 2 else "one block per CU", shape, tiling)
+        o, b = pair(orc, n=shape, problem=ic, **kw)
+        if tiling and n == 1:
+            tiling = (tiling[0], min(tiling[1], 46), tiling[2])  # the one-block kernel's LDS limit
+        b.set_xy_blocks_per_cu(n)
+        if tiling:
+            b.set_tiling(*tiling)
+        assert b.get_xy_blocks_per_cu() == n
+        assert b.get_tiling()[0] == 64
+        t_o, log_o = o.run(steps)
+        t_g, log_g, done = b.run(steps, mode=0)
+        assert done == steps
+        np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+        errP = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+        errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+        say(f"  max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+        assert np.all(errP <= TOL_L1), errP
+        assert np.all(errU <= TOL_L1), errU
+        b.close()
+
+
+def test_two_blocks_per_cu_refused_where_the_box_does_not_allow_it(orc):
+    """the switch refuses boxes whose strip edges are not all inside a block, and a one-block
+    kernel whose y chunk would exceed its LDS; mixed precision reports one block"""
+    from mhdturbulence_amd.capi import MhdError
+    base = dict(energy_flux=1, gam=5.0 / 3.0)
+    for shape, bc in (((80, 8, 8), KH_BC), ((256, 8, 8), (3, 3, 2, 2, 1, 1)), ((128, 8, 9), KH_BC),
+                      ((512, 8, 8), KH_BC)):
+        b = Box(Config(nx=shape[0], ny=shape[1], nz=shape[2], bc=bc, **base))
+        with pytest.raises(MhdError):
+            b.set_xy_blocks_per_cu(2)
+        assert b.get_xy_blocks_per_cu() == 1
+        b.close()
+    b = Box(Config(nx=256, ny=64, nz=8, bc=KH_BC, **base))
+    b.set_xy_blocks_per_cu(2)
+    b.set_tiling(64, 64, 8)
+    with pytest.raises(MhdError):
+        b.set_xy_blocks_per_cu(1)
+    assert b.get_xy_blocks_per_cu() == 2
+    b.set_tiling(32, 16, 8)
+    assert b.get_xy_blocks_per_cu() == 1   # 32-cell strips: the one-block kernel
+    b.close()
+    b = Box(Config(nx=256, ny=16, nz=8, bc=KH_BC, precision=2, **base))
+    b.set_xy_blocks_per_cu(2)
+    assert b.get_xy_blocks_per_cu() == 1
+    b.close()
+
+
 def test_tmax_stop_matches_reference_loop(orc):
     o, b = pair(orc, n=(16, 16, 16))
     dt0 = o.control_timestep()
