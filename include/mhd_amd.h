@@ -237,22 +237,23 @@ int mhd_batch_finish_box(mhd_batch* batch, int box_id, double* diag4, double* P_
 /* tile shape of a single box's fused kernels: sw 16/32/64 (x-strip width), 1 <= ly <= ny
  * (y chunk; also at most what the xy kernel's LDS for strip-edge faces allows per block:
  * FP64 46, 23, 11 rows (160 KiB less its window rings) and mixed 64, 32, 16 rows (64 KiB)
- * for sw 64, 32, 16; 64 rows where the FP64 march runs two blocks per CU with its strip
- * edges shared in the block: sw 64, nx 64, 128 or 256, x periodic), 1 <= lz <= min(nz, 64)
+ * for sw 64, 32, 16; 64 rows where the FP64 march runs two wavefronts per SIMD with its
+ * strip edges shared in the block, mhd_set_xy_waves_per_simd), 1 <= lz <= min(nz, 64)
  * (z chunk) */
 int mhd_set_tiling(mhd_ctx* ctx, int sw, int ly, int lz);
 int mhd_get_tiling(mhd_ctx* ctx, int* sw, int* ly, int* lz);
 
 /* FP64 xy kernel of a single box's fused step (no counterpart in the reference: a tuning
- * switch beside mhd_set_tiling).  n = 2: two blocks per CU with the strip edges shared in the
- * block, which needs sw 64, nx 64, 128 or 256, x periodic and, for nx 64 / 128, nz a multiple
- * of 4 / 2 (else MHD_ERR_ARG); n = 1: one block per CU with the strip-edge pre-pass, which
- * needs ly within that kernel's LDS limit (mhd_set_tiling); n = 0: the default rule (two
- * blocks where the box allows it and that launch has more blocks than the GPU has CUs).  The
- * results agree to round-off (the strip-edge faces are computed by different code).  Mixed
- * precision ignores it.  mhd_get_xy_blocks_per_cu gives the kernel the next step takes. */
-int mhd_set_xy_blocks_per_cu(mhd_ctx* ctx, int n);
-int mhd_get_xy_blocks_per_cu(mhd_ctx* ctx, int* n);
+ * switch beside mhd_set_tiling).  n = 2: up to two wavefronts per SIMD, the strip edges
+ * shared within each block and no strip-edge pre-pass, which needs sw 64, x periodic and
+ * nx = 64, 128, 192, 256 or 512 (for nx 64 / 128, nz a multiple of 4 / 2; else
+ * MHD_ERR_ARG); n = 1: one wavefront per SIMD with the strip-edge pre-pass, which needs ly
+ * within that kernel's LDS limit (mhd_set_tiling); n = 0: the default rule (n = 2 where the
+ * box allows it and that launch has more wavefronts than the GPU has SIMDs).  The results
+ * agree to round-off (the strip-edge faces are computed by different code).  Mixed precision
+ * ignores it.  mhd_get_xy_waves_per_simd gives the kernel the next step takes. */
+int mhd_set_xy_waves_per_simd(mhd_ctx* ctx, int n);
+int mhd_get_xy_waves_per_simd(mhd_ctx* ctx, int* n);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ EXPORTS = (
     "mhd_batch_create", "mhd_batch_destroy", "mhd_batch_box_bytes", "mhd_batch_free_bytes", "mhd_batch_set_tiling",
     "mhd_batch_add_box", "mhd_batch_upload_state", "mhd_batch_remove_box", "mhd_batch_step",
     "mhd_batch_last_step_ms", "mhd_batch_box_state", "mhd_batch_poll", "mhd_batch_finish_box",
-    "mhd_set_tiling", "mhd_get_tiling", "mhd_set_xy_blocks_per_cu", "mhd_get_xy_blocks_per_cu",
+    "mhd_set_tiling", "mhd_get_tiling", "mhd_set_xy_waves_per_simd", "mhd_get_xy_waves_per_simd",
 )
 
 
@@ -118,8 +118,8 @@ def load(path: str | None = None):
         "mhd_batch_finish_box": (C.c_int, [vp, C.c_int, dp, dp]),
         "mhd_set_tiling": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mhd_get_tiling": (C.c_int, [vp, ip, ip, ip]),
-        "mhd_set_xy_blocks_per_cu": (C.c_int, [vp, C.c_int]),
-        "mhd_get_xy_blocks_per_cu": (C.c_int, [vp, ip]),
+        "mhd_set_xy_waves_per_simd": (C.c_int, [vp, C.c_int]),
+        "mhd_get_xy_waves_per_simd": (C.c_int, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

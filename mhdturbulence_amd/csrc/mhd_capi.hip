@@ -869,16 +869,16 @@ int mhd_set_tiling(mhd_ctx* c, int sw, int ly, int lz) {
   return MHD_OK;
 }
 
-int mhd_set_xy_blocks_per_cu(mhd_ctx* c, int n) {
+int mhd_set_xy_waves_per_simd(mhd_ctx* c, int n) {
   if (!c) return fail(MHD_ERR_ARG, "null context");
-  if (n < 0 || n > 2) return fail(MHD_ERR_ARG, "xy blocks per CU: 0 (default rule), 1 or 2");
+  if (n < 0 || n > 2) return fail(MHD_ERR_ARG, "xy wavefronts per SIMD: 0 (default rule), 1 or 2");
   CHKRET(set_dev(c));
   CHKRET(ensure_fused(c));
   mhd::FusedArgs a = c->fa;
   if (n == 2) {
     if (!mhd::fused_xy_two_blocks_ok(a))
-      return fail(MHD_ERR_ARG, "xy blocks per CU: two blocks need sw 64, nx 64, 128 or 256, x periodic and, "
-                               "for nx 64 / 128, nz a multiple of 4 / 2");
+      return fail(MHD_ERR_ARG, "xy wavefronts per SIMD: two need sw 64, x periodic, nx 64, 128, 192, 256 or 512 "
+                               "and, for nx 64 / 128, nz a multiple of 4 / 2");
     a.duo = 1;
   } else if (n == 1) {
     a.duo = 0;
@@ -888,15 +888,15 @@ int mhd_set_xy_blocks_per_cu(mhd_ctx* c, int n) {
     a.duo = d.duo;
   }
   if (a.ly > mhd::fused_ly_cap(a))
-    return fail(MHD_ERR_ARG, "xy blocks per CU: the y chunk " + std::to_string(a.ly) + " is above " +
+    return fail(MHD_ERR_ARG, "xy wavefronts per SIMD: the y chunk " + std::to_string(a.ly) + " is above " +
                                  std::to_string(mhd::fused_ly_cap(a)) +
-                                 " (LDS of the one-block kernel); set a shorter tiling first");
+                                 " (LDS of the one-wavefront kernel); set a shorter tiling first");
   c->fa = a;
   drop_graph(c);
   return MHD_OK;
 }
 
-int mhd_get_xy_blocks_per_cu(mhd_ctx* c, int* n) {
+int mhd_get_xy_waves_per_simd(mhd_ctx* c, int* n) {
   if (!c || !n) return fail(MHD_ERR_ARG, "null argument");
   CHKRET(set_dev(c));
   CHKRET(ensure_fused(c));

@@ -331,22 +331,35 @@ __host__ __device__ inline bool xy_share(const FusedArgs& a) {
   return WIDE && MHD_XY_SHARE && MHD_XY_COOP != 2 && sizeof(R) == 8 && a.ns % WAVES == 0;
 }
 
-// Two FP64 blocks per CU (MHD_XY_DUO=1): where every strip edge of a block has its
-// neighbour in the block (full-wave strips, x periodic, a block holding one or more whole
-// x-rows: nx = 64, 128 or 256), the FP64 march takes the block-shared edges and no
-// pre-pass, and its window ring drops the row that the carried registers already hold
-// (3 slots: rows j, j+1 and the upper state of row j-1; row j-1 is the carry).  A block
-// then takes 77.5 KB of LDS, so two blocks share a CU: two wavefronts per SIMD, each
-// covering the other's barrier, LDS and memory waits.
+// Two wavefronts per SIMD for the FP64 march (MHD_XY_DUO=1): where every strip edge of a
+// block has its neighbour in the block (full-wave strips, x periodic, a block holding one or
+// more whole x-rows), the FP64 march takes the block-shared edges and no pre-pass, and its
+// window ring drops the row that the carried registers already hold (3 slots: rows j, j+1
+// and the upper state of row j-1; row j-1 is the carry).  A wavefront then takes 19.4 KB of
+// LDS instead of 24 KiB plus pre-pass slots, so a CU holds up to eight of them: two per SIMD,
+// each covering the other's barrier, LDS and memory waits.  Blocks hold 4 wavefronts for
+// nx = 64, 128, 256 (4, 2 or 1 whole x-rows; two blocks per CU) and one whole x-row of
+// nx / 64 wavefronts for nx = 192 (two blocks, six wavefronts per CU) and 512 (one block of
+// eight).  At nx = 384 one block of six ran 3 % slower than the one-wavefront kernel (its
+// SIMDs hold 2, 2, 1 and 1 wavefronts, all behind the same barriers), so it is not used.
 #ifndef MHD_XY_DUO
 #define MHD_XY_DUO 1
 #endif
 constexpr int WIN_RING3 = 3 * NPRIM * 64;
-__host__ __device__ inline bool xy_duo(const FusedArgs& a) {
-  if (!(MHD_XY_DUO && MHD_XY_RING && MHD_XY_CARRY && !a.mixed && a.sw == 64 && a.g.nx % 64 == 0)) return false;
-  if (!(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return false;
-  // the block's waves: the four strips of one x-row, or 4/ns whole x-rows of one y chunk
-  return a.ns == WAVES || ((a.ns == 1 || a.ns == 2) && a.g.nz % (WAVES / a.ns) == 0);
+// wavefronts per block of the two-wavefronts-per-SIMD march for a box, 0 where it does not apply
+__host__ __device__ inline int xy_duo_waves(const FusedArgs& a) {
+  if (!(MHD_XY_DUO && MHD_XY_RING && MHD_XY_CARRY && !a.mixed && a.sw == 64 && a.g.nx % 64 == 0)) return 0;
+  if (!(a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC)) return 0;
+  // four strips of one x-row, or 4/ns whole x-rows of one y chunk
+  if (a.ns == WAVES || ((a.ns == 1 || a.ns == 2) && a.g.nz % (WAVES / a.ns) == 0)) return WAVES;
+  // one whole x-row
+  if (a.ns == 3 || a.ns == 8) return a.ns;
+  return 0;
+}
+__host__ __device__ inline bool xy_duo(const FusedArgs& a) { return xy_duo_waves(a) > 0; }
+// static LDS of one block of the march: window rings and edge exchange
+__host__ __device__ constexpr size_t xy_duo_lds(int bw) {
+  return (size_t)bw * (WIN_RING3 + XY_EDGE_R + NCONS) * sizeof(double);
 }
 static_assert((WAVES + 1) * (NCONS + 2 * NPRIM) * sizeof(double) <= WAVES * 2 * (NCONS + NPRIM) * sizeof(double),
               "the shared pre-pass slots fit the block's per-wave slots");
@@ -363,9 +376,10 @@ __host__ __device__ constexpr int xy_edge_len(int per_wave) {
 // registers to the Riemann solves.  FP32 keeps the window in registers.
 // ACC32: the divergence is stored in FP32 (FusedArgs::acc32), fixed at compile time so the
 // other kernels carry nothing of it
-// DUO: the two-blocks-per-CU FP64 march (xy_duo holds): block-shared edges on both sides of
+// DUO: the two-wavefronts-per-SIMD FP64 march (xy_duo holds): block-shared edges on both sides of
 // every strip, 3-slot ring
-template <int EFLUX, typename R, bool WIDE, bool ACC32 = false, bool DUO = false>
+// BW: wavefronts per block
+template <int EFLUX, typename R, bool WIDE, bool ACC32 = false, bool DUO = false, int BW = WAVES>
 __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, int lane, char* lds, R* ring, R* edgeP,
                                         double* edgeF, int wv) {
   static_assert(!DUO || (WIDE && sizeof(R) == 8 && MHD_XY_RING && MHD_XY_CARRY), "DUO: FP64 full-wave strips");
@@ -413,7 +427,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   // 4/ns whole x-rows of the same y chunk (ns = 1 or 2 and nz a multiple of 4/ns), so all of
   // its waves march the same rows and take the same barriers
   const bool coop = DUO || xy_coop<R, WIDE>(a);
-  const int rw = min(a.ns, WAVES);  // strips of one x-row in the block
+  const int rw = min(a.ns, BW);  // strips of one x-row in the block
   const int pos = wv % rw, grp = wv - pos;
   const bool wrap = coop && a.ns <= WAVES && a.bc[0] == BC_PERIODIC && a.bc[1] == BC_PERIODIC;
   const bool coop_l = DUO || (coop && (pos > 0 || wrap));
@@ -721,17 +735,20 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs 
                           xedgeP, xedgeF, wv);
 }
 
-// FP64 march at two blocks per CU (xy_duo): static LDS only, 4 x 18 KiB of window rings and
-// the edge exchange, 77.5 KB per block
-template <int EFLUX>
-__global__ __launch_bounds__(256, 2) void k_fused_xy_duo(FusedArgs a) {
-  __shared__ double xring[WAVES * WIN_RING3];
-  __shared__ double xedgeP[WAVES * XY_EDGE_R];
-  __shared__ double xedgeF[WAVES * NCONS];
+// FP64 march at two wavefronts per SIMD (xy_duo): blocks of BW wavefronts, static LDS only
+// (BW x 18 KiB of window rings and the edge exchange: 77.5 KB for BW = 4, 155 KB for 8)
+template <int EFLUX, int BW>
+__global__ __launch_bounds__(64 * BW, 2) void k_fused_xy_duo(FusedArgs a) {
+  __shared__ double xring[BW * WIN_RING3];
+  __shared__ double xedgeP[BW * XY_EDGE_R];
+  __shared__ double xedgeF[BW * NCONS];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
-  xy_body<EFLUX, double, true, false, true>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63, nullptr,
-                                            xring + wv * WIN_RING3, xedgeP, xedgeF, wv);
+  xy_body<EFLUX, double, true, false, true, BW>(a, a.stage, blockIdx.x * BW + wv, threadIdx.x & 63, nullptr,
+                                                xring + wv * WIN_RING3, xedgeP, xedgeF, wv);
 }
+static_assert(xy_duo_lds(8) <= XY_LDS_PER_CU && 2 * xy_duo_lds(4) <= XY_LDS_PER_CU &&
+                  2 * xy_duo_lds(3) <= XY_LDS_PER_CU,
+              "two-wavefront march blocks fit the CU's LDS");
 
 // ---------------------------------------------------------------- z + update + recovery + reduction
 // The z kernels also keep the raw conserved state of the last three planes loaded: the old
@@ -1008,9 +1025,13 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 template <int EFLUX, typename R, bool ACC32>
 void launch_xy_t(const FusedArgs& a, hipStream_t s) {
   if constexpr (sizeof(R) == 8) {
-    if (a.duo && xy_duo(a)) {
-      hipLaunchKernelGGL((k_fused_xy_duo<EFLUX>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES), 0, s, a);
-      return;
+    const int bw = a.duo ? xy_duo_waves(a) : 0;
+    const dim3 grid(fused_blocks_xy(a)), block(64 * bw);
+    switch (bw) {
+      case 3: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 3>), grid, block, 0, s, a); return;
+      case 4: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 4>), grid, block, 0, s, a); return;
+      case 8: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 8>), grid, block, 0, s, a); return;
+      default: break;
     }
   }
   if (xy_wide(a))
@@ -1137,15 +1158,18 @@ bool fused_configure(FusedArgs& a, bool alone) {
     return std::max(std::min(n, 8), len);
   };
   const long long slots = a.mixed ? 2048 : 1024;
-  // the two-blocks-per-CU FP64 march (xy_duo) holds two wavefronts per SIMD and has no
+  // the two-wavefronts-per-SIMD FP64 march (xy_duo) has no
   // pre-pass slots to cap its chunk (64 rows: one per lane of its row constants)
   const long long strips = (long long)a.ns * ((a.g.nz + 64 / best - 1) / (64 / best));
   bool duo = alone && xy_duo(a);
   if (duo) {
-    // only where its chunks give more blocks than the chip has CUs: a smaller launch
-    // leaves CUs idle either way and pays the block barriers for nothing (64^3: 7 % slower)
-    a.ly = split(a.g.ny, 64, strips, 2048);
-    duo = strips * ((a.g.ny + a.ly - 1) / a.ly) > (long long)WAVES * 256;
+    // wavefronts a CU holds: whole blocks within its LDS, at most two per SIMD
+    const int bw = xy_duo_waves(a);
+    const long long per_cu = std::min<long long>(8, bw * (long long)(XY_LDS_PER_CU / xy_duo_lds(bw)));
+    a.ly = split(a.g.ny, 64, strips, 256 * per_cu);
+    // only where its chunks give more wavefronts than the chip has SIMDs: a smaller launch
+    // leaves SIMDs idle either way and pays the block barriers for nothing (64^3: 7 % slower)
+    duo = strips * ((a.g.ny + a.ly - 1) / a.ly) > 1024;
   }
   a.duo = duo ? 1 : 0;
   if (!duo) {
@@ -1160,7 +1184,8 @@ bool fused_configure(FusedArgs& a, bool alone) {
 int fused_blocks_xy(const FusedArgs& a) {
   const int spw = 64 / a.sw;
   const long long waves = (long long)a.ns * ((a.g.nz + spw - 1) / spw) * ((a.g.ny + a.ly - 1) / a.ly);
-  return (int)cdiv(waves, WAVES);
+  const int bw = (a.duo && xy_duo(a)) ? xy_duo_waves(a) : WAVES;
+  return (int)cdiv(waves, bw);
 }
 
 int fused_blocks_z(const FusedArgs& a) {

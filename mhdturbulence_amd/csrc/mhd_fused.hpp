@@ -42,8 +42,8 @@ struct FusedArgs {
   // U1 -> U0 averaged with the U0 it replaces; the state is in U0 after every iteration, and
   // only the second stage reduces the next iteration's dt and c_h
   int stage;
-  // 1: the FP64 xy march runs two blocks per CU (set by fused_configure for a box run alone
-  // whose strips and boundaries allow it; batches keep 0)
+  // 1: the FP64 xy march runs up to two wavefronts per SIMD (set by fused_configure for a box
+  // run alone whose strips and boundaries allow it; batches keep 0)
   int duo;
 };
 
@@ -62,10 +62,10 @@ size_t fused_xy_lds(const FusedArgs& a);
 size_t fused_xy_lds_max(int mixed);
 int fused_max_ly(int sw, int mixed);
 // the longest y chunk the single-box xy launch takes for a box's strip width (a.sw, a.ns,
-// a.duo set): fused_max_ly, or 64 for the two-blocks-per-CU FP64 march, which has no
+// a.duo set): fused_max_ly, or 64 for the two-wavefronts-per-SIMD FP64 march, which has no
 // pre-pass slots
 int fused_ly_cap(const FusedArgs& a);
-// whether the FP64 xy march can run two blocks per CU at a box's strip width and boundaries,
+// whether the FP64 xy march can run two wavefronts per SIMD at a box's strip width and boundaries,
 // and whether the next single-box launch does (a.duo set and allowed)
 bool fused_xy_two_blocks_ok(const FusedArgs& a);
 bool fused_xy_two_blocks(const FusedArgs& a);

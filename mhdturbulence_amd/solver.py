@@ -221,14 +221,15 @@ class Box:
         """x-strip width (16/32/64) and y / z march chunk lengths of the fused kernels"""
         capi.check(self.L.mhd_set_tiling(self.h, sw, ly, lz))
 
-    def set_xy_blocks_per_cu(self, n: int):
-        """FP64 xy kernel: 2 = two blocks per CU with block-shared strip edges (where the box
-        allows it), 1 = one block per CU with the strip-edge pre-pass, 0 = the default rule"""
-        capi.check(self.L.mhd_set_xy_blocks_per_cu(self.h, n))
+    def set_xy_waves_per_simd(self, n: int):
+        """FP64 xy kernel: 2 = up to two wavefronts per SIMD with block-shared strip edges
+        (where the box allows it), 1 = one wavefront per SIMD with the strip-edge pre-pass,
+        0 = the default rule"""
+        capi.check(self.L.mhd_set_xy_waves_per_simd(self.h, n))
 
-    def get_xy_blocks_per_cu(self) -> int:
+    def get_xy_waves_per_simd(self) -> int:
         n = C.c_int()
-        capi.check(self.L.mhd_get_xy_blocks_per_cu(self.h, C.byref(n)))
+        capi.check(self.L.mhd_get_xy_waves_per_simd(self.h, C.byref(n)))
         return n.value
 
     def get_tiling(self) -> tuple[int, int, int]:

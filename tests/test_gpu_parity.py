@@ -333,22 +333,24 @@ def test_block_cooperative_strip_edges(orc, shape, bc):
     ((64, 6, 8), 12, None),         # one strip per row, four rows per block
     ((64, 40, 24), 40, (64, 16, 24)),   # three y chunks: halo rows between chunks
     ((128, 70, 12), 20, (64, 64, 12)),  # 64-row chunks, the longest, and a 6-row last one
+    ((192, 10, 6), 12, None),       # one x-row of three strips per block
+    ((512, 12, 4), 12, (64, 5, 4)), # one x-row of eight strips per block, three y chunks
 ])
-def test_fp64_two_blocks_per_cu(orc, shape, steps, tiling):
-    """The FP64 xy march at two blocks per CU (strip edges shared in the block, 3-slot window
+def test_fp64_two_waves_per_simd(orc, shape, steps, tiling):
+    """The FP64 xy march at two wavefronts per SIMD (strip edges shared in the block, 3-slot window
     ring with row j-1 taken from the carried registers), forced on boxes below the default
-    rule's block count, against the oracle; then the one-block kernel on the same box"""
+    rule's wavefront count, against the oracle; then the one-wavefront kernel on the same box"""
     kw = dict(bc=KH_BC, energy_flux=1, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), grid_shift=0)
     ic = dict(bx0=0.1, a=0.06, amp=0.04)
     for n in (2, 1):
-        say("two blocks per CU" if n == 2 else "one block per CU", shape, tiling)
+        say("two wavefronts per SIMD" if n == 2 else "one wavefront per SIMD", shape, tiling)
         o, b = pair(orc, n=shape, problem=ic, **kw)
         if tiling and n == 1:
-            tiling = (tiling[0], min(tiling[1], 46), tiling[2])  # the one-block kernel's LDS limit
-        b.set_xy_blocks_per_cu(n)
+            tiling = (tiling[0], min(tiling[1], 46), tiling[2])  # the one-wavefront kernel's LDS limit
+        b.set_xy_waves_per_simd(n)
         if tiling:
             b.set_tiling(*tiling)
-        assert b.get_xy_blocks_per_cu() == n
+        assert b.get_xy_waves_per_simd() == n
         assert b.get_tiling()[0] == 64
         t_o, log_o = o.run(steps)
         t_g, log_g, done = b.run(steps, mode=0)
@@ -362,30 +364,30 @@ def test_fp64_two_blocks_per_cu(orc, shape, steps, tiling):
         b.close()
 
 
-def test_two_blocks_per_cu_refused_where_the_box_does_not_allow_it(orc):
-    """the switch refuses boxes whose strip edges are not all inside a block, and a one-block
-    kernel whose y chunk would exceed its LDS; mixed precision reports one block"""
+def test_two_waves_per_simd_refused_where_the_box_does_not_allow_it(orc):
+    """the switch refuses boxes whose strip edges are not all inside a block, and a
+    one-wavefront kernel whose y chunk would exceed its LDS; mixed precision reports one"""
     from mhdturbulence_amd.capi import MhdError
     base = dict(energy_flux=1, gam=5.0 / 3.0)
     for shape, bc in (((80, 8, 8), KH_BC), ((256, 8, 8), (3, 3, 2, 2, 1, 1)), ((128, 8, 9), KH_BC),
-                      ((512, 8, 8), KH_BC)):
+                      ((320, 8, 8), KH_BC), ((384, 8, 4), KH_BC)):
         b = Box(Config(nx=shape[0], ny=shape[1], nz=shape[2], bc=bc, **base))
         with pytest.raises(MhdError):
-            b.set_xy_blocks_per_cu(2)
-        assert b.get_xy_blocks_per_cu() == 1
+            b.set_xy_waves_per_simd(2)
+        assert b.get_xy_waves_per_simd() == 1
         b.close()
     b = Box(Config(nx=256, ny=64, nz=8, bc=KH_BC, **base))
-    b.set_xy_blocks_per_cu(2)
+    b.set_xy_waves_per_simd(2)
     b.set_tiling(64, 64, 8)
     with pytest.raises(MhdError):
-        b.set_xy_blocks_per_cu(1)
-    assert b.get_xy_blocks_per_cu() == 2
+        b.set_xy_waves_per_simd(1)
+    assert b.get_xy_waves_per_simd() == 2
     b.set_tiling(32, 16, 8)
-    assert b.get_xy_blocks_per_cu() == 1   # 32-cell strips: the one-block kernel
+    assert b.get_xy_waves_per_simd() == 1   # 32-cell strips: the one-wavefront kernel
     b.close()
     b = Box(Config(nx=256, ny=16, nz=8, bc=KH_BC, precision=2, **base))
-    b.set_xy_blocks_per_cu(2)
-    assert b.get_xy_blocks_per_cu() == 1
+    b.set_xy_waves_per_simd(2)
+    assert b.get_xy_waves_per_simd() == 1
     b.close()
 
 
