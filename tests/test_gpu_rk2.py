@@ -94,6 +94,24 @@ def test_rk2_other_boundaries(orc, bc):
         assert np.all(errP <= TOL_L1) and np.all(errU <= TOL_L1), (mode, errP, errU)
 
 
+@pytest.mark.parametrize("shape,bc", [((256, 12, 8), (1, 1, 2, 2, 1, 1)), ((128, 10, 6), (1, 1, 3, 3, 3, 3)),
+                                      ((192, 9, 4), (1, 1, 2, 2, 2, 2))])
+def test_rk2_two_waves_per_simd(orc, shape, bc):
+    """both stages on the FP64 xy march at two wavefronts per SIMD (block-shared strip edges),
+    forced on small boxes, with reflecting and outflow walls in y and z"""
+    say("rk2 two wavefronts per SIMD", shape, bc)
+    o, b = _pair(orc, shape, bc=bc)
+    b.set_xy_waves_per_simd(2)
+    assert b.get_xy_waves_per_simd() == 2
+    t_o, log_o = o.run(20)
+    t_g, log_g, done = b.run(20, mode=0)
+    assert done == 20
+    np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+    errP, errU = _errs(o, b)
+    say("  relL1", errP.max(), errU.max())
+    assert errP.max() <= TOL_L1 and errU.max() <= TOL_L1
+
+
 def test_rk2_run_continues_across_calls_and_stops_at_tmax(orc):
     o, b = _pair(orc, (24, 24, 16))
     t1, _, d1 = b.run(7, mode=0)
