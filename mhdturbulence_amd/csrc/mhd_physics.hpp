@@ -403,31 +403,30 @@ __device__ __forceinline__ void hlld(const RState<R>& L, const RState<R>& Rt, RF
 #define MHD_HLLD_NEAR 1
 #endif
 
-// One interface state in the rotated frame, as HLLD reads it before the side is known
+// One interface state in the rotated frame, as HLLD reads it before the side is known.  The
+// total energy and the energy flux's magnetic factor are formed after the pick, for the
+// surviving side only (hlld_near), from e and |B|^2 kept here.
 template <typename R>
 struct RPrim {
-  R rho, vn, vv, vw, bn, bv, bw, E, pt, X, eb, cf, psi;  // eb: |B|^2 or v.B (energy flux)
+  R rho, vn, vv, vw, bn, bv, bw, e, bsq, pt, X, cf, psi;
 };
 
 template <int D, int EFLUX, typename R>
 __device__ __forceinline__ void prim_to_rprim(const R* q, R rhoinv, RPrim<R>& s) {
   constexpr int DF = (D + 1) % 3, DB = (D + 2) % 3;
-  const R rho = q[nden];
-  const R vsq = q[nve1] * q[nve1] + q[nve2] * q[nve2] + q[nve3] * q[nve3];
   const R bsq = q[nbm1] * q[nbm1] + q[nbm2] * q[nbm2] + q[nbm3] * q[nbm3];
   const R bn = q[nbm1 + D];
-  s.rho = rho;
+  s.rho = q[nden];
   s.vn = q[nve1 + D];
   s.vv = q[nve1 + DF];
   s.vw = q[nve1 + DB];
   s.bn = bn;
   s.bv = q[nbm1 + DF];
   s.bw = q[nbm1 + DB];
-  s.E = rho * q[nene] + R(0.5) * rho * vsq + R(0.5) * bsq;
+  s.e = q[nene];
+  s.bsq = bsq;
   s.pt = q[npre] + bsq / R(2);
   s.X = q[nst];
-  if constexpr (EFLUX == 0) s.eb = bsq;
-  else s.eb = q[nve1] * q[nbm1] + q[nve2] * q[nbm2] + q[nve3] * q[nbm3];
   const R css = q[ncsp] * q[ncsp];
   const R cts = css + bsq * rhoinv;
   s.cf = sq((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
@@ -439,7 +438,9 @@ __device__ __forceinline__ R pick(bool right, R l, R r) {
   return right ? r : l;
 }
 
-template <typename R>
+// D: direction of the face (the rotated slots map back to Cartesian components for the
+// energy's sums, in the order Prim2ConsD forms them); EFLUX as prim_to_state
+template <int D, int EFLUX, typename R>
 __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt, RFlux<R>& nf) {
   const R eps = R(1.0e-30);
   const R tiny = R(1.0e-20);
@@ -510,8 +511,22 @@ __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt,
   const R ro = pick(right, rol, ror), vx = pick(right, vxl, vxr);
   const R vy = pick(right, L.vv, Rt.vv), vz = pick(right, L.vw, Rt.vw);
   const R bn = pick(right, L.bn, Rt.bn), by = pick(right, L.bv, Rt.bv), bz = pick(right, L.bw, Rt.bw);
-  const R ee = pick(right, L.E, Rt.E), pt = pick(right, ptl, ptr);
-  const R X = pick(right, L.X, Rt.X), eb = pick(right, L.eb, Rt.eb);
+  const R e = pick(right, L.e, Rt.e), bsq = pick(right, L.bsq, Rt.bsq), pt = pick(right, ptl, ptr);
+  const R X = pick(right, L.X, Rt.X);
+  // its total energy and energy-flux factor (KH/mhd.cpp:1036-1050) from the Cartesian
+  // components: slot s of the rotated frame holds component (s + D) % 3
+  R vc[3], bc[3];
+  vc[D] = vx;
+  vc[(D + 1) % 3] = vy;
+  vc[(D + 2) % 3] = vz;
+  bc[D] = bn;
+  bc[(D + 1) % 3] = by;
+  bc[(D + 2) % 3] = bz;
+  const R vsq = vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2];
+  const R ee = ro * e + R(0.5) * ro * vsq + R(0.5) * bsq;
+  R eb;
+  if constexpr (EFLUX == 0) eb = bsq;
+  else eb = vc[0] * bc[0] + vc[1] * bc[1] + vc[2] * bc[2];
   const R sd = pick(right, sdl, sdr), isdm = pick(right, isdml, isdmr);
   const R maxs = pick(right, stl.maxs, str.maxs), mins = pick(right, stl.mins, str.mins);
   const R rost = pick(right, stl.ro, str.ro), vyst = pick(right, stl.vy, str.vy), vzst = pick(right, stl.vz, str.vz);
@@ -573,7 +588,7 @@ __device__ __forceinline__ void riemann_flux(const R* ql, const R* qr, double ch
   RPrim<R> sl, sr;
   prim_to_rprim<D, EFLUX, R>(ql, rinvl, sl);
   prim_to_rprim<D, EFLUX, R>(qr, rinvr, sr);
-  hlld_near<R>(sl, sr, nf);
+  hlld_near<D, EFLUX, R>(sl, sr, nf);
   const double bnl = double(sl.bn), bnr = double(sr.bn);
 #else
   RState<R> sl, sr;
