@@ -391,6 +391,44 @@ def test_two_waves_per_simd_refused_where_the_box_does_not_allow_it(orc):
     b.close()
 
 
+@pytest.mark.parametrize("variant", ["two_waves", "one_wave", "staged"])
+def test_step_on_nonuniform_grid(orc, variant):
+    """The whole step on a stretched grid (random cell widths, a different draw per axis):
+    every kernel takes each cell's own 1/dx, 1/dy, 1/dz and CFL widths.  The two-wavefront
+    march runs 64-row y chunks (64, 64 and 12 rows), whose rows take their 1/dy from one lane
+    each; on a uniform grid a wrong row or lane there would not show."""
+    import divb_fields as dvf
+    n = (256, 140, 6)
+    kw = dict(bc=KH_BC, energy_flux=1, gam=5.0 / 3.0, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5), grid_shift=0)
+    say("stretched grid", n, variant)
+    o = orc.Oracle(nx=n[0], ny=n[1], nz=n[2], **kw)
+    o.generate_grid()
+    for ax, seed in enumerate((11, 12, 13)):
+        f = dvf.nonuniform_faces(n[ax], -0.5, 1.0, seed)
+        o.grid(ax + 1)[:] = f
+        o.grid(ax + 1, True)[:] = dvf.centres(f)
+    o.generate_problem_shear(bx0=0.1, a=0.06, amp=0.04)
+    b = Box(Config(nx=n[0], ny=n[1], nz=n[2], **kw))
+    b.set_grid(o.grid(1), o.grid(1, True), o.grid(2), o.grid(2, True), o.grid(3), o.grid(3, True))
+    b.upload_prim(o.P)
+    b.upload_cons(o.U)
+    if variant == "two_waves":
+        b.set_xy_waves_per_simd(2)
+        b.set_tiling(64, 64, 6)
+        assert b.get_tiling() == (64, 64, 6) and b.get_xy_waves_per_simd() == 2
+    elif variant == "one_wave":
+        b.set_xy_waves_per_simd(1)
+    t_o, log_o = o.run(15)
+    t_g, log_g, done = b.run(15, mode=1 if variant == "staged" else 0)
+    assert done == 15
+    np.testing.assert_allclose(log_g, log_o, rtol=1e-12)
+    errP = rel_l1(o.interior(b.download_prim()), o.interior(o.P))
+    errU = rel_l1(o.interior(b.download_cons()), o.interior(o.U))
+    say(f"  max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+    assert np.all(errP <= TOL_L1), errP
+    assert np.all(errU <= TOL_L1), errU
+
+
 def test_tmax_stop_matches_reference_loop(orc):
     o, b = pair(orc, n=(16, 16, 16))
     dt0 = o.control_timestep()
