@@ -8,7 +8,10 @@ Requests (a shear-layer box of n^3 cells run to time t_end) arrive over time.  T
   batched launch per kernel per step (mhd_batch_*);
 * retires a box when it passes its t_end (each box stops on its own clock), returns its
   t-prof diagnostics (volume, mix, sqrt(<vy^2>), max|divB|) and frees its region for the
-  next request.
+  next request;
+* drops a request its client cancels (Request.cancel_at): still pending, it is never
+  admitted; live, its box is removed at the next chunk boundary and its slot and region
+  go to the next request.  The other boxes' results do not change.
 
 `serve()` replays a request stream against the wall clock and reports per-request latency
 (completion - arrival), p50 / p99 and aggregate cell-updates/s.  `serve_one_at_a_time()` is
@@ -32,6 +35,8 @@ class Request:
     t_end: float
     ic: dict
     arrival: float = 0.0
+    # seconds after the stream starts at which the client cancels the request (None: never)
+    cancel_at: float | None = None
 
 
 @dataclass
@@ -48,6 +53,7 @@ class Result:
     # non-finite CFL bound and stopped there, before its t_end
     status: int = 0
     prim: np.ndarray | None = None  # final primitive state (reference layout), if kept
+    cancelled: bool = False  # dropped at its cancel_at, before t_end (state as it stood)
 
     @property
     def latency(self) -> float:
@@ -110,9 +116,12 @@ def poisson_stream(rate: float, count: int, seed: int = 20261203, n_range=(32, 2
 
 
 def _summary(results: list, wall: float, cell_updates: int, gpu_s: float) -> dict:
-    lat = np.array([r.latency for r in results])
+    """latency percentiles over the completed requests; cancelled ones are counted apart"""
+    done = [r for r in results if not r.cancelled]
+    lat = np.array([r.latency for r in done] or [0.0])
     return {
-        "requests": len(results),
+        "requests": len(done),
+        "cancelled": len(results) - len(done),
         "wall_s": wall,
         "gpu_s": gpu_s,
         "cell_updates": cell_updates,
@@ -187,11 +196,28 @@ class Server:
         results: list[Result] = []
         cell_updates = 0
         gpu_ms = 0.0
+        def cancelled(r: Request, now: float) -> bool:
+            return r.cancel_at is not None and now >= r.cancel_at
+
         while len(results) < len(reqs):
             now = clock() - t0
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 pending.append(reqs[nxt])
                 nxt += 1
+            # cancellations: pending requests are dropped, live boxes removed with their state
+            for r in [r for r in pending if cancelled(r, now)]:
+                pending.remove(r)
+                results.append(Result(r.rid, r.n, r.arrival, now, now, 0, 0.0, cancelled=True))
+            for bid in [b for b, (r, _) in live.items() if cancelled(r, now)]:
+                r, adm = live.pop(bid)
+                st = self.batch.box_state(bid)
+                diag, prim = self.batch.finish_box(bid, want_prim=keep_prim)
+                self.batch.remove_box(bid)
+                cell_updates += st["steps"] * r.n ** 3
+                results.append(Result(r.rid, r.n, r.arrival, adm, clock() - t0, st["steps"], st["time"],
+                                      diag.tolist(), st["status"], prim, cancelled=True))
+                if log:
+                    log(f"cancelled rid={r.rid} n={r.n} after {st['steps']} steps")
             # admission in arrival order while a slot and a region of the footprint are free
             while pending:
                 r = pending[0]
@@ -203,7 +229,7 @@ class Server:
                 live[bid] = (r, clock() - t0)
                 pending.pop(0)
             if not live:
-                if nxt < len(reqs):
+                if nxt < len(reqs) and not pending:
                     time.sleep(max(0.0, reqs[nxt].arrival - (clock() - t0)))
                 continue
             gpu_ms += self.batch.step(self.chunk)

@@ -105,3 +105,37 @@ def test_served_requests_match_oracle(orc, server):
             assert err[0] <= 1e-6 and err[5] <= 1e-6 and np.all(err <= 1e-4), err
         else:
             assert err.max() <= 1e-10, (r, err)
+
+
+def test_cancelled_requests_leave_the_others_unchanged():
+    """a live request cancelled after its first chunk of steps and a pending one cancelled
+    before admission: both are returned as cancelled, their slot and region are freed, and
+    the other requests end with the same bits as when served without them.  The clock is a
+    counter (one unit per reading), so the cancellations fall at fixed points of the loop."""
+    def counter():
+        n = [0]
+
+        def clock():
+            n[0] += 1
+            return float(n[0])
+        return clock
+
+    keep = [Request(0, 16, 0.006, request_params(0, 9), 0.0), Request(1, 24, 0.005, request_params(1, 9), 0.0)]
+    live_cancel = Request(2, 32, 10.0, request_params(2, 9), 0.0, cancel_at=4.5)
+    pending_cancel = Request(3, 16, 10.0, request_params(3, 9), 0.0, cancel_at=4.5)
+    srv = Server(Config(**BASE), max_boxes=3, arena_bytes=1 << 30, chunk_steps=3)
+    out = srv.serve(keep + [live_cancel, pending_cancel], clock=counter())
+    avail, total = srv.batch.free_bytes()
+    srv.close()
+    assert avail == total
+    assert out["requests"] == 2 and out["cancelled"] == 2
+    res = {r.rid: r for r in out["results"]}
+    assert res[2].cancelled and res[2].steps == 3 and res[2].sim_time < live_cancel.t_end
+    assert res[3].cancelled and res[3].steps == 0
+    ref_srv = Server(Config(**BASE), max_boxes=3, arena_bytes=1 << 30, chunk_steps=3)
+    ref = {r.rid: r for r in ref_srv.serve(keep, clock=counter())["results"]}
+    ref_srv.close()
+    for k in (0, 1):
+        assert not res[k].cancelled and res[k].sim_time > keep[k].t_end
+        assert res[k].steps == ref[k].steps and res[k].sim_time == ref[k].sim_time
+        assert res[k].diagnostics == ref[k].diagnostics
