@@ -346,6 +346,17 @@ __host__ __device__ inline bool xy_share(const FusedArgs& a) {
 #define MHD_XY_DUO 1
 #endif
 constexpr int WIN_RING3 = 3 * NPRIM * 64;
+
+// Second Runge-Kutta stage in the W form (MHD_RK2_W=1, FP64 divergence storage): the stage's
+// xy kernel, which is bound by instruction issue and has HBM bandwidth to spare, reads U^n
+// of each cell and stores W = U^n - dt (dFx/dx + dFy/dy) in place of the divergence (the psi
+// component's divergence term scaled by the cell's damping factor), and the z kernel forms
+// (W + U* - dt dFz/dz, psi damped) / 2.  That is (U^n + E(U*)) / 2 summed in another order,
+// and the z kernel no longer reads U^n: 240 instead of 320 bytes per cell.  Precision 2 keeps
+// the plain form (its divergence is stored in FP32, W could not be).
+#ifndef MHD_RK2_W
+#define MHD_RK2_W 1
+#endif
 // wavefronts per block of the two-wavefronts-per-SIMD march for a box, 0 where it does not apply
 __host__ __device__ inline int xy_duo_waves(const FusedArgs& a) {
   if (!(MHD_XY_DUO && MHD_XY_RING && MHD_XY_CARRY && !a.mixed && a.sw == 64 && a.g.nx % 64 == 0)) return 0;
@@ -378,11 +389,12 @@ __host__ __device__ constexpr int xy_edge_len(int per_wave) {
 // other kernels carry nothing of it
 // DUO: the two-wavefronts-per-SIMD FP64 march (xy_duo holds): block-shared edges on both sides of
 // every strip, 3-slot ring
-// BW: wavefronts per block
-template <int EFLUX, typename R, bool WIDE, bool ACC32 = false, bool DUO = false, int BW = WAVES>
+// BW: wavefronts per block; W2: the second Runge-Kutta stage in the W form (MHD_RK2_W)
+template <int EFLUX, typename R, bool WIDE, bool ACC32 = false, bool DUO = false, int BW = WAVES, bool W2 = false>
 __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, int lane, char* lds, R* ring, R* edgeP,
                                         double* edgeF, int wv) {
   static_assert(!DUO || (WIDE && sizeof(R) == 8 && MHD_XY_RING && MHD_XY_CARRY), "DUO: FP64 full-wave strips");
+  static_assert(!W2 || !ACC32, "the W form stores FP64");
   const StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -418,9 +430,15 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   const int j1 = min(j0 + a.ly, NGH + g.ny);
   const int nrows = j1 - j0;
   const double xinv = 1.0 / (a.x1a[i + 1] - a.x1a[i]);
+  // W2: dt of this step, the cell's x and plane's z width, and dy of the chunk's rows one per
+  // lane (as yinv_l), for the psi damping factor of DampPsi (KH/mhd.cpp:1625-1631)
+  const double dt2 = W2 ? a.cfl * bits2d(sc->red[iter % 3][0]) : 0.0;
+  const double dxa2 = W2 ? a.x1a[i + 1] - a.x1a[i] : 0.0;
+  const double dza2 = W2 ? a.x3a[k + 1] - a.x3a[k] : 0.0;
   // 1/dy of the chunk's rows, row j0 + r in lane r (a.ly <= 64), read back per row by index
   const double yinv_l =
       pin_lanes(1.0 / (a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)]));
+  const double dy2_l = pin_lanes(W2 ? a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)] : 0.0);
   double* ldsF = reinterpret_cast<double*>(lds);
   R* ldsN = reinterpret_cast<R*>(lds + (size_t)SPW * 2 * a.ly * NCONS * sizeof(double));
   // block-uniform: a block holds 4 neighbouring strips of one x-row (ns a multiple of 4), or
@@ -531,6 +549,7 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   load_prim_image<1, R>(U, g, a.bc, k, j0 - 1, i, a.gam, P0);
   load_prim_image<1, R>(U, g, a.bc, k, j0, i, a.gam, Pp);
   const Cols Uc = make_cols(U, vs);
+  const Cols U0c = make_cols(W2 ? a.U0 : U, vs);  // W2: U^n (stage 2 reads U* from U1)
   if (coop) {
     if (lane == 0 || lane == 63) {
       R* e = edge_cell(wv, j0, lane == 63);
@@ -646,6 +665,11 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
         }
       }
     }
+    // W2: U^n of the cell completed in this pass, issued by every lane on every pass (row
+    // >= j0 - 2 >= 0 is in the array; unused when not stored), after the x faces so that it
+    // is held only through the y face
+    double un0[W2 ? NCONS : 1];
+    if constexpr (W2) load_raw(U0c, g.cell(k, row, i), un0);
     const double yinv = lane_row(yinv_l, max(row - j0, 0));
     // y states of row j and the y face j-1/2
     R up[NPRIM], dn[NPRIM];
@@ -689,6 +713,12 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
       double d[NCONS];
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) d[m] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
+      if constexpr (W2) {
+        const double dya = lane_row(dy2_l, max(row - j0, 0));
+        const double damp = 1.0 - dt2 * (a.alphabp * chg / fmin(dxa2, fmin(dya, dza2)));
+#pragma unroll
+        for (int m = 0; m < NCONS; ++m) d[m] = un0[m] - dt2 * (m == mbps ? d[m] * damp : d[m]);
+      }
       if constexpr (ACC32) store_acc32(reinterpret_cast<float*>(a.acc), vs, g.cell(k, row, i), d);
       else store_acc(a.acc, g.cell(k, row, i), d);
     }
@@ -723,27 +753,27 @@ __host__ __device__ inline bool xy_wide(const FusedArgs& a) { return MHD_XY_DPP 
 template <typename R>
 __host__ __device__ constexpr int xy_ring_len() { return (MHD_XY_RING && sizeof(R) == 8) ? WAVES * WIN_RING : 1; }
 
-template <int EFLUX, typename R, bool WIDE, bool ACC32>
+template <int EFLUX, typename R, bool WIDE, bool ACC32, bool W2 = false>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_xy(FusedArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ R xring[xy_ring_len<R>()];
   __shared__ R xedgeP[xy_edge_len<R>(XY_EDGE_R)];
   __shared__ double xedgeF[xy_edge_len<R>(NCONS)];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
-  xy_body<EFLUX, R, WIDE, ACC32>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
+  xy_body<EFLUX, R, WIDE, ACC32, false, WAVES, W2>(a, a.stage, blockIdx.x * WAVES + wv, threadIdx.x & 63,
                           smem + wv * xy_wave_lds<R>(a.sw, a.ly), xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0),
                           xedgeP, xedgeF, wv);
 }
 
 // FP64 march at two wavefronts per SIMD (xy_duo): blocks of BW wavefronts, static LDS only
 // (BW x 18 KiB of window rings and the edge exchange: 77.5 KB for BW = 4, 155 KB for 8)
-template <int EFLUX, int BW>
+template <int EFLUX, int BW, bool W2 = false>
 __global__ __launch_bounds__(64 * BW, 2) void k_fused_xy_duo(FusedArgs a) {
   __shared__ double xring[BW * WIN_RING3];
   __shared__ double xedgeP[BW * XY_EDGE_R];
   __shared__ double xedgeF[BW * NCONS];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform over the wave
-  xy_body<EFLUX, double, true, false, true, BW>(a, a.stage, blockIdx.x * BW + wv, threadIdx.x & 63, nullptr,
+  xy_body<EFLUX, double, true, false, true, BW, W2>(a, a.stage, blockIdx.x * BW + wv, threadIdx.x & 63, nullptr,
                                                 xring + wv * WIN_RING3, xedgeP, xedgeF, wv);
 }
 static_assert(xy_duo_lds(8) <= XY_LDS_PER_CU && 2 * xy_duo_lds(4) <= XY_LDS_PER_CU &&
@@ -764,6 +794,9 @@ constexpr int Z_RAW = 3 * NCONS * 64;
 template <int EFLUX, typename R, int STAGE, bool ACC32 = false>
 __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
   constexpr bool AVG = STAGE == 2;
+  // the W form of the second stage (MHD_RK2_W): the divergence buffer holds W (the xy
+  // kernel's U^n - dt dF_xy), so U^n is not read here
+  constexpr bool WFORM = AVG && !ACC32 && MHD_RK2_W;
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
   const Geom& g = a.g;
@@ -868,8 +901,8 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     if constexpr (ACC32) load_acc32(reinterpret_cast<const float*>(a.acc), vs, g.cell(kk, j, i), ac);
     else load_acc(a.acc, g.cell(kk, j, i), ac);
     // the second stage's U^n of cell kk, loaded under the same rule as the divergence
-    double un0[AVG ? NCONS : 1];
-    if constexpr (AVG) load_raw(Uoc, g.cell(kk, j, i), un0);
+    double un0[(AVG && !WFORM) ? NCONS : 1];
+    if constexpr (AVG && !WFORM) load_raw(Uoc, g.cell(kk, j, i), un0);
     R up[NPRIM], dn[NPRIM], upprev[NPRIM];
     if constexpr (PRING) {
       R Pm[NPRIM], P0[NPRIM], Pp[NPRIM];
@@ -908,12 +941,18 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     if (upd) {
       double u[NCONS];
 #pragma unroll
-      for (int m = 0; m < NCONS; ++m) u[m] = uo[m] - dt * (ac[m] + (Fz[m] - Fzprev[m]) * zinv);
+      for (int m = 0; m < NCONS; ++m) {
+        if constexpr (WFORM) u[m] = uo[m] - dt * ((Fz[m] - Fzprev[m]) * zinv);
+        else u[m] = uo[m] - dt * (ac[m] + (Fz[m] - Fzprev[m]) * zinv);
+      }
       // DampPsi (KH/mhd.cpp:1625-1631)
       const double dhl = fmin(dxa, fmin(dya, dza));
       const double taui = a.alphabp * chg / dhl;
       u[mbps] = u[mbps] * (1.0 - dt * taui);
-      if constexpr (AVG) {
+      if constexpr (WFORM) {
+#pragma unroll
+        for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (ac[m] + u[m]);  // ac holds W
+      } else if constexpr (AVG) {
 #pragma unroll
         for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (un0[m] + u[m]);
       }
@@ -970,7 +1009,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_fused_z(FusedArgs a
 // Many boxes in one launch: tab[block] = (box, block index within that box).  Every box's
 // block count is a whole number of blocks, so a block never straddles two boxes and the
 // box descriptor read is uniform over the block.  stage: FusedArgs::stage for every box.
-template <int EFLUX, typename R, bool ACC32>
+template <int EFLUX, typename R, bool ACC32, bool W2 = false>
 __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const FusedArgs* __restrict__ boxes,
                                                                      const int2* __restrict__ tab, int stage) {
   extern __shared__ __align__(16) char smem[];
@@ -983,8 +1022,11 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_xy(const Fuse
   char* lds = smem + wv * xy_wave_lds<R>(a.sw, a.ly);
   R* ring = xring + (xy_ring_len<R>() > 1 ? wv * WIN_RING : 0);
   if (xy_wide(a))
-    xy_body<EFLUX, R, true, ACC32>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
-  else xy_body<EFLUX, R, false, ACC32>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP, xedgeF, wv);
+    xy_body<EFLUX, R, true, ACC32, false, WAVES, W2>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP,
+                                                     xedgeF, wv);
+  else
+    xy_body<EFLUX, R, false, ACC32, false, WAVES, W2>(a, stage, e.y * WAVES + wv, threadIdx.x & 63, lds, ring, xedgeP,
+                                                      xedgeF, wv);
 }
 
 template <int EFLUX, typename R, int STAGE, bool ACC32>
@@ -1022,26 +1064,36 @@ __global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int*
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
-template <int EFLUX, typename R, bool ACC32>
-void launch_xy_t(const FusedArgs& a, hipStream_t s) {
+template <int EFLUX, typename R, bool ACC32, bool W2>
+void launch_xy_w(const FusedArgs& a, hipStream_t s) {
   if constexpr (sizeof(R) == 8) {
     const int bw = a.duo ? xy_duo_waves(a) : 0;
     const dim3 grid(fused_blocks_xy(a)), block(64 * bw);
     switch (bw) {
-      case 3: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 3>), grid, block, 0, s, a); return;
-      case 4: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 4>), grid, block, 0, s, a); return;
-      case 8: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 8>), grid, block, 0, s, a); return;
+      case 3: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 3, W2>), grid, block, 0, s, a); return;
+      case 4: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 4, W2>), grid, block, 0, s, a); return;
+      case 8: hipLaunchKernelGGL((k_fused_xy_duo<EFLUX, 8, W2>), grid, block, 0, s, a); return;
       default: break;
     }
   }
   if (xy_wide(a))
-    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true, ACC32>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, true, ACC32, W2>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
                        fused_xy_lds(a), s, a);
   else
-    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, false, ACC32>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
+    hipLaunchKernelGGL((k_fused_xy<EFLUX, R, false, ACC32, W2>), dim3(fused_blocks_xy(a)), dim3(64 * WAVES),
                        fused_xy_lds(a), s, a);
 }
-
+// the second Runge-Kutta stage takes the W form where the divergence is stored in FP64
+template <int EFLUX, typename R, bool ACC32>
+void launch_xy_t(const FusedArgs& a, hipStream_t s) {
+  if constexpr (!ACC32 && MHD_RK2_W) {
+    if (a.stage == 2) {
+      launch_xy_w<EFLUX, R, ACC32, true>(a, s);
+      return;
+    }
+  }
+  launch_xy_w<EFLUX, R, ACC32, false>(a, s);
+}
 template <int EFLUX, typename R, bool ACC32>
 void launch_z_t(const FusedArgs& a, hipStream_t s) {
   const dim3 grid(fused_blocks_z(a)), block(64 * WAVES);
@@ -1060,6 +1112,13 @@ struct BatchLaunch {
 
 template <int EFLUX, typename R, bool ACC32>
 void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
+  if constexpr (!ACC32 && MHD_RK2_W) {
+    if (b.stage == 2) {
+      hipLaunchKernelGGL((k_batch_xy<EFLUX, R, ACC32, true>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes,
+                         b.tab, b.stage);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_batch_xy<EFLUX, R, ACC32>), dim3(b.nblocks), dim3(64 * WAVES), b.lds, s, b.boxes, b.tab,
                      b.stage);
 }
