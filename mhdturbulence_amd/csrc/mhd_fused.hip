@@ -349,11 +349,12 @@ constexpr int WIN_RING3 = 3 * NPRIM * 64;
 
 // Second Runge-Kutta stage in the W form (MHD_RK2_W=1, FP64 divergence storage): the stage's
 // xy kernel, which is bound by instruction issue and has HBM bandwidth to spare, reads U^n
-// of each cell and stores W = U^n - dt (dFx/dx + dFy/dy) in place of the divergence (the psi
-// component's divergence term scaled by the cell's damping factor), and the z kernel forms
-// (W + U* - dt dFz/dz, psi damped) / 2.  That is (U^n + E(U*)) / 2 summed in another order,
-// and the z kernel no longer reads U^n: 240 instead of 320 bytes per cell.  Precision 2 keeps
-// the plain form (its divergence is stored in FP32, W could not be).
+// of each cell and stores W = U^n - dt (dFx/dx + dFy/dy) in place of the divergence for the
+// nine variables psi damping leaves alone, and the z kernel forms (W + U* - dt dFz/dz) / 2:
+// (U^n + E(U*)) / 2 summed in another order.  psi keeps the plain form (its divergence stored,
+// U^n of psi read by the z kernel), so the damping stays where it was.  The z kernel reads
+// 248 instead of 320 bytes per cell.  Precision 2 keeps the plain form (its divergence is
+// stored in FP32, W could not be).
 #ifndef MHD_RK2_W
 #define MHD_RK2_W 1
 #endif
@@ -430,15 +431,10 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
   const int j1 = min(j0 + a.ly, NGH + g.ny);
   const int nrows = j1 - j0;
   const double xinv = 1.0 / (a.x1a[i + 1] - a.x1a[i]);
-  // W2: dt of this step, the cell's x and plane's z width, and dy of the chunk's rows one per
-  // lane (as yinv_l), for the psi damping factor of DampPsi (KH/mhd.cpp:1625-1631)
-  const double dt2 = W2 ? a.cfl * bits2d(sc->red[iter % 3][0]) : 0.0;
-  const double dxa2 = W2 ? a.x1a[i + 1] - a.x1a[i] : 0.0;
-  const double dza2 = W2 ? a.x3a[k + 1] - a.x3a[k] : 0.0;
+  const double dt2 = W2 ? a.cfl * bits2d(sc->red[iter % 3][0]) : 0.0;  // W2: dt of this step
   // 1/dy of the chunk's rows, row j0 + r in lane r (a.ly <= 64), read back per row by index
   const double yinv_l =
       pin_lanes(1.0 / (a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)]));
-  const double dy2_l = pin_lanes(W2 ? a.x2a[j0 + min(lane, nrows - 1) + 1] - a.x2a[j0 + min(lane, nrows - 1)] : 0.0);
   double* ldsF = reinterpret_cast<double*>(lds);
   R* ldsN = reinterpret_cast<R*>(lds + (size_t)SPW * 2 * a.ly * NCONS * sizeof(double));
   // block-uniform: a block holds 4 neighbouring strips of one x-row (ns a multiple of 4), or
@@ -669,7 +665,12 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     // >= j0 - 2 >= 0 is in the array; unused when not stored), after the x faces so that it
     // is held only through the y face
     double un0[W2 ? NCONS : 1];
-    if constexpr (W2) load_raw(U0c, g.cell(k, row, i), un0);
+    if constexpr (W2) {
+      const unsigned ob = boff(g.cell(k, row, i));
+#pragma unroll
+      for (int m = 0; m < NCONS; ++m)
+        if (m != mbps) un0[m] = *(const __attribute__((address_space(1))) double*)(U0c.p[m] + ob);
+    }
     const double yinv = lane_row(yinv_l, max(row - j0, 0));
     // y states of row j and the y face j-1/2
     R up[NPRIM], dn[NPRIM];
@@ -714,10 +715,9 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) d[m] = ax[m] + (Fy[m] - Fyprev[m]) * yinv;
       if constexpr (W2) {
-        const double dya = lane_row(dy2_l, max(row - j0, 0));
-        const double damp = 1.0 - dt2 * (a.alphabp * chg / fmin(dxa2, fmin(dya, dza2)));
 #pragma unroll
-        for (int m = 0; m < NCONS; ++m) d[m] = un0[m] - dt2 * (m == mbps ? d[m] * damp : d[m]);
+        for (int m = 0; m < NCONS; ++m)
+          if (m != mbps) d[m] = un0[m] - dt2 * d[m];
       }
       if constexpr (ACC32) store_acc32(reinterpret_cast<float*>(a.acc), vs, g.cell(k, row, i), d);
       else store_acc(a.acc, g.cell(k, row, i), d);
@@ -795,7 +795,7 @@ template <int EFLUX, typename R, int STAGE, bool ACC32 = false>
 __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* ring, double* raw) {
   constexpr bool AVG = STAGE == 2;
   // the W form of the second stage (MHD_RK2_W): the divergence buffer holds W (the xy
-  // kernel's U^n - dt dF_xy), so U^n is not read here
+  // kernel's U^n - dt dF_xy) for every variable but psi, so only U^n of psi is read here
   constexpr bool WFORM = AVG && !ACC32 && MHD_RK2_W;
   StepScalars* sc = a.sc;
   if (sc->stopped) return;
@@ -901,8 +901,9 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
     if constexpr (ACC32) load_acc32(reinterpret_cast<const float*>(a.acc), vs, g.cell(kk, j, i), ac);
     else load_acc(a.acc, g.cell(kk, j, i), ac);
     // the second stage's U^n of cell kk, loaded under the same rule as the divergence
-    double un0[(AVG && !WFORM) ? NCONS : 1];
-    if constexpr (AVG && !WFORM) load_raw(Uoc, g.cell(kk, j, i), un0);
+    double un0[AVG ? NCONS : 1];
+    if constexpr (WFORM) un0[mbps] = *(const __attribute__((address_space(1))) double*)(Uoc.p[mbps] + oc);
+    else if constexpr (AVG) load_raw(Uoc, g.cell(kk, j, i), un0);
     R up[NPRIM], dn[NPRIM], upprev[NPRIM];
     if constexpr (PRING) {
       R Pm[NPRIM], P0[NPRIM], Pp[NPRIM];
@@ -942,7 +943,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
       double u[NCONS];
 #pragma unroll
       for (int m = 0; m < NCONS; ++m) {
-        if constexpr (WFORM) u[m] = uo[m] - dt * ((Fz[m] - Fzprev[m]) * zinv);
+        if (WFORM && m != mbps) u[m] = uo[m] - dt * ((Fz[m] - Fzprev[m]) * zinv);
         else u[m] = uo[m] - dt * (ac[m] + (Fz[m] - Fzprev[m]) * zinv);
       }
       // DampPsi (KH/mhd.cpp:1625-1631)
@@ -951,7 +952,7 @@ __device__ __forceinline__ void z_body(const FusedArgs& a, int w, int lane, R* r
       u[mbps] = u[mbps] * (1.0 - dt * taui);
       if constexpr (WFORM) {
 #pragma unroll
-        for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (ac[m] + u[m]);  // ac holds W
+        for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * ((m == mbps ? un0[m] : ac[m]) + u[m]);  // ac holds W
       } else if constexpr (AVG) {
 #pragma unroll
         for (int m = 0; m < NCONS; ++m) u[m] = 0.5 * (un0[m] + u[m]);
