@@ -95,8 +95,15 @@ __device__ __forceinline__ void recip4(R a, R b, R c, R d, R& ia, R& ib, R& ic, 
 // within an ulp of the IEEE root.  It skips the denormal/huge-argument rescaling, which the
 // step's arguments (sound and Alfven speeds squared, densities) never need; x = 0 returns 0
 // and x < 0 returns NaN as sqrt does.  FP32 uses the hardware square root.
+// MHD_SQRT_FLOOR=1 (default): where x may be 0, the reciprocal root is taken of x + 1e-300
+// instead of selecting x = 0 afterwards.  The root itself is x y refined, which is exactly
+// x (+0 or -0) when x is, and for x >= 2e-284 the floor is under half an ulp of x, so
+// x + 1e-300 == x and the result is the one without the floor.
 #ifndef MHD_FAST_SQRT
 #define MHD_FAST_SQRT 1
+#endif
+#ifndef MHD_SQRT_FLOOR
+#define MHD_SQRT_FLOOR 1
 #endif
 template <typename R>
 __device__ __forceinline__ R sq(R x) {
@@ -108,18 +115,23 @@ __device__ __forceinline__ R sq_pos(R x) {
   return sqrt(x);
 }
 #if MHD_FAST_SQRT
-__device__ __forceinline__ double sq_rsq(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
+// y: the reciprocal square root of x (or of x + a floor, see above)
+__device__ __forceinline__ double sq_rsq(double x, double y) {
   double g = x * y, h = 0.5 * y;
   const double r = fma(-h, g, 0.5);
   g = fma(g, r, g);
   h = fma(h, r, h);
   return fma(fma(-g, g, x), h, g);
 }
+__device__ __forceinline__ double sq_rsq(double x) { return sq_rsq(x, __builtin_amdgcn_rsq(x)); }
 template <>
 __device__ __forceinline__ double sq<double>(double x) {
-  const double g = sq_rsq(x);
-  return x == 0.0 ? x : g;
+  if constexpr (MHD_SQRT_FLOOR) {
+    return sq_rsq(x, __builtin_amdgcn_rsq(x + 1.0e-300));
+  } else {
+    const double g = sq_rsq(x);
+    return x == 0.0 ? x : g;
+  }
 }
 template <>
 __device__ __forceinline__ double sq_pos<double>(double x) {
@@ -167,8 +179,15 @@ __device__ __forceinline__ void sq_rsq_pos(float x, float& s, float& rs) {
 // relative, measured over 6.7e7 arguments), so after the Newton step h is within 2.2e-15
 // relative of the quotient, about 10 ulp (h enters the
 // states as q0 +- h, a fraction of the cell value).
+// MHD_LIM_FLOOR=1 (default, FP64): no select.  The reciprocal is taken of (dp + dm) + 1e-300,
+// equal to dp + dm wherever |dp + dm| >= 2e-284, and max(dp dm, 0) replaces the product, so
+// a pair of slopes of opposite sign (or a zero) gives a zero, which is -0 where dp + dm < 0:
+// q0 +- h is then q0 except that a q0 of -0 may come back as +0 (and the reverse).
 #ifndef MHD_LIM_DIV
 #define MHD_LIM_DIV 1
+#endif
+#ifndef MHD_LIM_FLOOR
+#define MHD_LIM_FLOOR 1
 #endif
 template <typename R>
 __device__ __forceinline__ R half_van_leer(R dp, R dm) {
@@ -176,6 +195,13 @@ __device__ __forceinline__ R half_van_leer(R dp, R dm) {
   R h;
   if constexpr (MHD_LIM_DIV && MHD_FAST_DIV && sizeof(R) == 8) {
     const R s = dp + dm;
+    if constexpr (MHD_LIM_FLOOR) {
+      // where dp dm <= 0, s may be 0: the reciprocal of s + 1e-300 stays finite and the
+      // product with max(dp dm, 0) = 0 is a zero
+      R r = __builtin_amdgcn_rcp(s + R(1.0e-300));
+      r = fma(fma(-s, r, R(1)), r, r);
+      return fmax(prod, R(0)) * r;
+    }
     R r = __builtin_amdgcn_rcp(s);
     r = fma(fma(-s, r, R(1)), r, r);
     h = prod * r;
