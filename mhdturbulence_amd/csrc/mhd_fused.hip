@@ -696,8 +696,10 @@ __device__ __forceinline__ void xy_body(const FusedArgs& a, int stage, int w, in
     } else {
       cell_states<R>(Pm, P0, Pp, up, dn);
     }
-    double Fy[NCONS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j >= j0) riemann_flux<1, EFLUX, R>(upprev, dn, chg, hchg, Fy);
+    // the first pass (j = j0 - 1) solves a face against the zero state upprev starts as; its
+    // flux only becomes Fyprev of pass j0, which completes no row (its row j0 - 1 is a halo)
+    double Fy[NCONS];
+    riemann_flux<1, EFLUX, R>(upprev, dn, chg, hchg, Fy);
     // row j+2 enters before this pass's stores are issued: a wait for a load behind a
     // pending store would drain the store too (one counter for both on this architecture)
     R Pn[NPRIM];

@@ -465,6 +465,16 @@ template <typename R>
 __device__ __forceinline__ R pick(bool right, R l, R r) {
   return right ? r : l;
 }
+// pick of two values whose low 32 bits are zero (0, +-1 in FP64): one 32-bit select
+template <typename R>
+__device__ __forceinline__ R pick_hi(bool right, R l, R r) {
+  if constexpr (sizeof(R) == 8) {
+    const int hi = right ? __double2hiint(r) : __double2hiint(l);
+    return __hiloint2double(hi, 0);
+  } else {
+    return right ? r : l;
+  }
+}
 
 // D: direction of the face (the rotated slots map back to Cartesian components for the
 // energy's sums, in the order Prim2ConsD forms them); EFLUX as prim_to_state
@@ -556,7 +566,8 @@ __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt,
   if constexpr (EFLUX == 0) eb = bsq;
   else eb = vc[0] * bc[0] + vc[1] * bc[1] + vc[2] * bc[2];
   const R sd = pick(right, sdl, sdr), isdm = pick(right, isdml, isdmr);
-  const R maxs = pick(right, stl.maxs, str.maxs), mins = pick(right, stl.mins, str.mins);
+  // max(0, sign) and min(0, sign) of the side: 0 or +-1
+  const R maxs = pick_hi(right, stl.maxs, str.maxs), mins = pick_hi(right, stl.mins, str.mins);
   const R rost = pick(right, stl.ro, str.ro), vyst = pick(right, stl.vy, str.vy), vzst = pick(right, stl.vz, str.vz);
   const R byst = pick(right, stl.by, str.by), bzst = pick(right, stl.bz, str.bz);
   const R sqro = pick(right, sqrtrol, sqrtror), rsqro = pick(right, rsqrtrol, rsqrtror);
@@ -568,7 +579,6 @@ __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt,
   const R rxst = rost * vxst;
   const R ryst = rost * vyst;
   const R rzst = rost * vzst;
-  const R scst = rost * X;
   const R vdbst = vxst * bxs + vyst * byst + vzst * bzst;
   const R eest = maxs * (sd * ee - pt * vx + ptst * sm + bxs * (vx * bxs + vy * by + vz * bz - vdbst)) * isdm - mins * ee;
 
@@ -582,16 +592,16 @@ __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt,
   const R tdot = sm * bxs + vydst * bydst + vzdst * bzdst;
   const R eedst = eest + sqro * (sgn * signbx) * (vdbst - tdot) * maxs1;
 
-  // its outer and Alfven wave speeds, clipped to the side (KH/mhd.cpp:367-375)
-  const R sast = (sm + (sgn * abbx) * rsqro) * maxs1;
+  // its outer and Alfven wave speeds, clipped to the side (KH/mhd.cpp:367-375): the Alfven
+  // speed S*_side = (S_M + sgn |B_x| / sqrt(rho*_side)) maxs1 is clipped as sgn max(sgn S*, 0),
+  // max(S*, 0) on the right and min(S*, 0) on the left (a multiplication by sgn is exact)
   const R ms = right ? fmax(sr, R(0)) : fmin(sl, R(0));
-  const R mst = right ? fmax(sast, R(0)) : fmin(sast, R(0));
+  const R mst = sgn * fmax((sgn * sm + abbx * rsqro) * maxs1, R(0));
 
   // its physical flux (KH/mhd.cpp:1036-1076) and the flux of the side (KH/mhd.cpp:384-415)
   const R rvx = ro * vx;
   const R rvy = ro * vy;
   const R rvz = ro * vz;
-  const R sc = ro * X;
   nf.d = rvx + (rost - ro) * ms;
   nf.u = (ro * vx * vx - bn * bn + pt) + (rxst - rvx) * ms;
   nf.v = (ro * vy * vx - by * bn) + (ryst - rvy) * ms + (rydst - ryst) * mst;
@@ -599,7 +609,8 @@ __device__ __forceinline__ void hlld_near(const RPrim<R>& L, const RPrim<R>& Rt,
   nf.E = ((ee + pt) * vx - eb * bn) + (eest - ee) * ms + (eedst - eest) * mst;
   nf.bv = (by * vx - vy * bn) + (byst - by) * ms + (bydst - byst) * mst;
   nf.bw = (bz * vx - vz * bn) + (bzst - bz) * ms + (bzdst - bzst) * mst;
-  nf.sc = ro * X * vx + (scst - sc) * ms;
+  // the scalar's flux ro X vx + (rost X - ro X) ms is X times the density flux
+  nf.sc = X * nf.d;
 }
 
 // Interface flux in Cartesian conserved order (F[0..9]) from reconstructed left/right
