@@ -210,6 +210,15 @@ __device__ __forceinline__ R half_van_leer(R dp, R dm) {
   }
   return prod > R(0) ? h : R(0);
 }
+// The floored FP64 limiter as the factors of h = max(dp dm, 0) r, so that the caller forms
+// q0 +- h as two fused multiply-adds (one rounding each instead of two)
+constexpr bool LIM_FACTORS = MHD_LIM_DIV && MHD_FAST_DIV && MHD_LIM_FLOOR;
+__device__ __forceinline__ void half_van_leer_factors(double dp, double dm, double& pm, double& r) {
+  const double s = dp + dm;
+  r = __builtin_amdgcn_rcp(s + 1.0e-300);
+  r = fma(fma(-s, r, 1.0), r, r);
+  pm = fmax(dp * dm, 0.0);
+}
 
 // One interface state in the rotated frame: the quantities HLLD reads from the
 // 23-entry Riemann vector of KH/mhd.cpp:997-1076, kept as named registers.
@@ -455,9 +464,17 @@ __device__ __forceinline__ void prim_to_rprim(const R* q, R rhoinv, RPrim<R>& s)
   s.bsq = bsq;
   s.pt = q[npre] + bsq / R(2);
   s.X = q[nst];
-  const R css = q[ncsp] * q[ncsp];
-  const R cts = css + bsq * rhoinv;
-  s.cf = sq((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  if constexpr (sizeof(R) == 8 && MHD_FAST_SQRT && MHD_SQRT_FLOOR) {
+    // c_s^2 + 1e-300 in one rounding (c_s^2 itself for c_s^2 >= 2e-284): the outer root's
+    // argument is then at least css / 2 > 0 and needs no floor of its own
+    const R css = fma(q[ncsp], q[ncsp], R(1.0e-300));
+    const R cts = css + bsq * rhoinv;
+    s.cf = sq_pos((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  } else {
+    const R css = q[ncsp] * q[ncsp];
+    const R cts = css + bsq * rhoinv;
+    s.cf = sq((cts + sq(cts * cts - R(4) * css * bn * bn * rhoinv)) / R(2));
+  }
   s.psi = q[nbps];
 }
 
@@ -657,9 +674,16 @@ template <typename R>
 __device__ __forceinline__ void cell_states(const R* qm, const R* q0, const R* qp, R* to_right, R* to_left) {
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) {
-    const R h = half_van_leer(qp[n] - q0[n], q0[n] - qm[n]);
-    to_right[n] = q0[n] + h;
-    to_left[n] = q0[n] - h;
+    if constexpr (LIM_FACTORS && sizeof(R) == 8) {
+      double pm, r;
+      half_van_leer_factors(qp[n] - q0[n], q0[n] - qm[n], pm, r);
+      to_right[n] = fma(pm, r, q0[n]);
+      to_left[n] = fma(-pm, r, q0[n]);
+    } else {
+      const R h = half_van_leer(qp[n] - q0[n], q0[n] - qm[n]);
+      to_right[n] = q0[n] + h;
+      to_left[n] = q0[n] - h;
+    }
   }
 }
 
