@@ -13,7 +13,31 @@
 
 #include <algorithm>
 
+// Compilation units.  MHD_FUSED_UNIT=0 (default): the whole file.  The package builds it as
+// two units: 1, the host side and the FP32 kernels, and 2, the FP64 kernels and their
+// launches, which the build compiles with the instruction-level-parallelism scheduler
+// (-mllvm -amdgpu-sched-strategy=max-ilp: the FP64 march 1 % faster at 256^3, while the
+// FP32 kernels ran 4-5 % slower with it).
+#ifndef MHD_FUSED_UNIT
+#define MHD_FUSED_UNIT 0
+#endif
+
 namespace mhd {
+
+// one batched launch: the device box descriptors, the work table, its length, the dynamic
+// LDS of the xy kernel and the Runge-Kutta stage
+struct BatchLaunch {
+  const FusedArgs* boxes;
+  const int2* tab;
+  int nblocks;
+  size_t lds;
+  int stage;
+};
+// the FP64 launches (in unit 2, or in this unit when it is the whole file)
+void launch_xy_f64(const FusedArgs& a, hipStream_t s);
+void launch_z_f64(const FusedArgs& a, hipStream_t s);
+void launch_bxy_f64(const BatchLaunch& b, int eflux, hipStream_t s);
+void launch_bz_f64(const BatchLaunch& b, int eflux, hipStream_t s);
 
 namespace {
 
@@ -1044,6 +1068,7 @@ __global__ __launch_bounds__(256, MinWaves<R>::value) void k_batch_z(const Fused
                           zraw + wv * Z_RAW);
 }
 
+#if MHD_FUSED_UNIT != 2
 // step end of every live box (k_step_end with the fused ring offset), one thread per box
 __global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int* __restrict__ live, int nlive,
                                  double* __restrict__ dt_last) {
@@ -1064,6 +1089,7 @@ __global__ void k_batch_step_end(const FusedArgs* __restrict__ boxes, const int*
   if (sc->tmax > 0.0 && sc->time > sc->tmax) sc->stopped = 1;
   if (sc->halt) sc->stopped = 1;  // the state this step left holds a bad cell
 }
+#endif
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
@@ -1105,14 +1131,6 @@ void launch_z_t(const FusedArgs& a, hipStream_t s) {
   else hipLaunchKernelGGL((k_fused_z<EFLUX, R, 0, ACC32>), grid, block, 0, s, a);
 }
 
-struct BatchLaunch {
-  const FusedArgs* boxes;
-  const int2* tab;
-  int nblocks;
-  size_t lds;
-  int stage;
-};
-
 template <int EFLUX, typename R, bool ACC32>
 void launch_bxy_t(const BatchLaunch& b, hipStream_t s) {
   if constexpr (!ACC32 && MHD_RK2_W) {
@@ -1133,22 +1151,41 @@ void launch_bz_t(const BatchLaunch& b, hipStream_t s) {
   else hipLaunchKernelGGL((k_batch_z<EFLUX, R, 0, ACC32>), grid, block, 0, s, b.boxes, b.tab);
 }
 
-// prec: mhd_config.precision (0 FP64; 1 FP32 reconstruction + HLLD; 2 as 1 with the
+// FP32 launches: prec = mhd_config.precision (1 FP32 reconstruction + HLLD; 2 as 1 with the
 // divergence stored in FP32)
-#define MHD_DISPATCH_E(fn, E, prec, a, s)        \
-  do {                                          \
-    if ((prec) == 2) fn<E, float, true>(a, s);  \
-    else if (prec) fn<E, float, false>(a, s);   \
-    else fn<E, double, false>(a, s);            \
+#define MHD_DISPATCH_F32(fn, eflux, prec, a, s)                          \
+  do {                                                                   \
+    if ((eflux) == 0) {                                                  \
+      if ((prec) == 2) fn<0, float, true>(a, s);                         \
+      else fn<0, float, false>(a, s);                                    \
+    } else {                                                             \
+      if ((prec) == 2) fn<1, float, true>(a, s);                         \
+      else fn<1, float, false>(a, s);                                    \
+    }                                                                    \
   } while (0)
-#define MHD_DISPATCH2(fn, eflux, prec, a, s)               \
-  do {                                                     \
-    if ((eflux) == 0) MHD_DISPATCH_E(fn, 0, prec, a, s);   \
-    else MHD_DISPATCH_E(fn, 1, prec, a, s);                \
-  } while (0)
-#define MHD_DISPATCH(fn, a, s) MHD_DISPATCH2(fn, (a).eflux, ((a).mixed ? ((a).acc32 ? 2 : 1) : 0), a, s)
 
 }  // namespace
+
+#if MHD_FUSED_UNIT != 1
+void launch_xy_f64(const FusedArgs& a, hipStream_t s) {
+  if (a.eflux == 0) launch_xy_t<0, double, false>(a, s);
+  else launch_xy_t<1, double, false>(a, s);
+}
+void launch_z_f64(const FusedArgs& a, hipStream_t s) {
+  if (a.eflux == 0) launch_z_t<0, double, false>(a, s);
+  else launch_z_t<1, double, false>(a, s);
+}
+void launch_bxy_f64(const BatchLaunch& b, int eflux, hipStream_t s) {
+  if (eflux == 0) launch_bxy_t<0, double, false>(b, s);
+  else launch_bxy_t<1, double, false>(b, s);
+}
+void launch_bz_f64(const BatchLaunch& b, int eflux, hipStream_t s) {
+  if (eflux == 0) launch_bz_t<0, double, false>(b, s);
+  else launch_bz_t<1, double, false>(b, s);
+}
+#endif
+
+#if MHD_FUSED_UNIT != 2
 
 size_t fused_xy_lds_max(int mixed) {
   if (mixed) return FUSED_XY_LDS_MAX;
@@ -1258,18 +1295,31 @@ int fused_blocks_z(const FusedArgs& a) {
 
 void launch_batch_xy(const FusedArgs* boxes, const int2* tab, int nblocks, size_t lds, int eflux, int mixed,
                      hipStream_t s, int stage) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bxy_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, lds, stage}), s);
+  if (nblocks <= 0) return;
+  const BatchLaunch b{boxes, tab, nblocks, lds, stage};
+  if (mixed) MHD_DISPATCH_F32(launch_bxy_t, eflux, mixed, b, s);
+  else launch_bxy_f64(b, eflux, s);
 }
 void launch_batch_z(const FusedArgs* boxes, const int2* tab, int nblocks, int eflux, int mixed, hipStream_t s,
                     int stage) {
-  if (nblocks > 0) MHD_DISPATCH2(launch_bz_t, eflux, mixed, (BatchLaunch{boxes, tab, nblocks, 0, stage}), s);
+  if (nblocks <= 0) return;
+  const BatchLaunch b{boxes, tab, nblocks, 0, stage};
+  if (mixed) MHD_DISPATCH_F32(launch_bz_t, eflux, mixed, b, s);
+  else launch_bz_f64(b, eflux, s);
 }
 void launch_batch_step_end(const FusedArgs* boxes, const int* live, int nlive, double* dt_last, hipStream_t s) {
   if (nlive > 0)
     hipLaunchKernelGGL(k_batch_step_end, dim3(cdiv(nlive, 256)), dim3(256), 0, s, boxes, live, nlive, dt_last);
 }
 
-void launch_fused_xy(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_xy_t, a, s); }
-void launch_fused_z(const FusedArgs& a, hipStream_t s) { MHD_DISPATCH(launch_z_t, a, s); }
+void launch_fused_xy(const FusedArgs& a, hipStream_t s) {
+  if (a.mixed) MHD_DISPATCH_F32(launch_xy_t, a.eflux, a.acc32 ? 2 : 1, a, s);
+  else launch_xy_f64(a, s);
+}
+void launch_fused_z(const FusedArgs& a, hipStream_t s) {
+  if (a.mixed) MHD_DISPATCH_F32(launch_z_t, a.eflux, a.acc32 ? 2 : 1, a, s);
+  else launch_z_f64(a, s);
+}
+#endif  // MHD_FUSED_UNIT != 2
 
 }  // namespace mhd
