@@ -63,8 +63,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/fused_mixed_256cube_pmc.md.  Keyed by (n, mode, precision, kernel); a workload
 # without a measurement reports null.
 MEASURED_TRAFFIC = {
-    (256, 0, 0, "fused_xy"): 694794.1 * 1024 * 2.0 + 1310774.8 * 1024 * 1.0,
-    (256, 0, 0, "fused_z_update"): 1373790.1 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 0, "fused_xy"): 694769.6 * 1024 * 2.0 + 1310721.7 * 1024 * 1.0,
+    (256, 0, 0, "fused_z_update"): 1373216.0 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
     (256, 0, 2, "fused_xy"): 719849.3 * 1024 * 2.0 + 655360.6 * 1024 * 1.0,
     (256, 0, 2, "fused_z_update"): 1112416.2 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
 }
@@ -72,8 +72,8 @@ MEASURED_TRAFFIC = {
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
 # MI355X in the same profile runs; keyed like MEASURED_TRAFFIC.
 MEASURED_VALU = {
-    (256, 0, 0, "fused_xy"): 349.2e6 * 64,
-    (256, 0, 0, "fused_z_update"): 224.4e6 * 64,
+    (256, 0, 0, "fused_xy"): 327.3e6 * 64,
+    (256, 0, 0, "fused_z_update"): 209.7e6 * 64,
     (256, 0, 2, "fused_xy"): 295.6e6 * 64,
     (256, 0, 2, "fused_z_update"): 209.6e6 * 64,
 }

@@ -206,6 +206,80 @@ def test_wide_range_states(orc):
         b2.close()
 
 
+def _floor_state(o, gam):
+    """states that take the limiter's and the roots' zero cases: half the box uniform
+    (every slope pair 0, 0: dp + dm = 0), the other half a checkerboard (every cell a local
+    extremum in each direction: dp = -dm, dp + dm = 0 with dp dm < 0), signed zeros in v2
+    and psi, and in the uniform half c_s = 1, rho = 1, B = (1, 0, 0), where the fast speed's
+    inner root of an x face is sqrt((c_s^2 + B^2/rho)^2 - 4 c_s^2 B_x^2/rho) = sqrt(0)"""
+    P = o.P
+    K, J, _, I = P.shape
+    k, j, i = np.meshgrid(np.arange(K), np.arange(J), np.arange(I), indexing="ij")
+    chk = ((i + j + k) % 2).astype(float)
+    half = i < I // 2
+    rho = np.where(half, 1.0, 1.0 + 0.5 * chk)
+    e0 = 1.0 / (gam * (gam - 1.0))
+    P[:, :, 0] = rho
+    P[:, :, 1] = np.where(half, 0.0, 0.3 * chk - 0.1)
+    P[:, :, 2] = np.where(half, -0.0, 0.2 * (1.0 - 2.0 * chk))
+    P[:, :, 3] = 0.0
+    P[:, :, 4] = e0
+    P[:, :, 5] = (gam - 1.0) * rho * e0
+    P[:, :, 6] = np.where(half, 1.0, np.sqrt(gam * (gam - 1.0) * e0))
+    P[:, :, 7] = np.where(half, 1.0, 0.5 * chk)
+    P[:, :, 8] = 0.0
+    P[:, :, 9] = 0.0
+    P[:, :, 10] = np.where(half, 0.0, -0.0)
+    P[:, :, 11] = np.where(half, 0.0, chk)
+    o.prim_to_cons_all()
+
+
+def test_limiter_and_root_zero_cases(orc):
+    """the limiter's zero slope sums and opposite-sign slope pairs, signed zeros and a fast
+    speed whose inner root is of exactly 0 (the cases the limiter's and the roots' 1e-300
+    floors stand in for selects): every face of every direction, then three fused and three
+    staged steps, against the oracle"""
+    say("limiter and root zero cases 20x18x16")
+    n = (20, 18, 16)
+    kw = dict(energy_flux=1, gam=5.0 / 3.0, bc=KH_BC)
+    o, b = pair(orc, n=n, problem="kh", upload=False, **kw)
+    _floor_state(o, kw["gam"])
+    b.upload_prim(o.P)
+    b.upload_cons(o.U)
+    dt_o, dt_g = o.control_timestep(), b.control_timestep()
+    assert dt_g == pytest.approx(dt_o, rel=1e-15)
+    o.set_boundary()
+    b.set_boundary_condition()
+    ch_o, ch_g = o.evaluate_ch(), b.evaluate_ch()
+    assert ch_g == pytest.approx(ch_o, rel=1e-15)
+    for d in (1, 2, 3):
+        o.numerical_flux(d, ch_o)
+        b.get_numerical_flux(d, ch_o)
+        Fo, Fg = o.F(d), b.download_flux(d)
+        sl = [slice(2, -3)] * 3
+        sl[2 - (d - 1)] = slice(2, -2)
+        s = (sl[0], sl[1], slice(None), sl[2])
+        assert np.isfinite(Fg[s]).all()
+        scale = np.abs(Fo[s]).max(axis=2, keepdims=True) + 1e-300
+        err = (np.abs(Fg[s] - Fo[s]) / scale).max()
+        assert err < 1e-13, (d, err)
+    b.close()
+    for mode in (0, 1):
+        o2, b2 = pair(orc, n=n, problem="kh", upload=False, **kw)
+        _floor_state(o2, kw["gam"])
+        b2.upload_prim(o2.P)
+        b2.upload_cons(o2.U)
+        t_o, log_o = o2.run(3)
+        t_g, log_g, done = b2.run(3, mode=mode)
+        assert done == 3
+        np.testing.assert_allclose(log_g, log_o, rtol=1e-13)
+        errP = rel_l1(o2.interior(b2.download_prim()), o2.interior(o2.P))
+        errU = rel_l1(o2.interior(b2.download_cons()), o2.interior(o2.U))
+        say(f"  mode {mode}: max relL1 P={errP.max():.3e} U={errU.max():.3e}")
+        assert np.all(errP < 1e-13) and np.all(errU < 1e-13), (mode, errP, errU)
+        b2.close()
+
+
 def _run_parity(orc, n, steps, mode, problem="kh", **kw):
     o, b = pair(orc, n=n, problem=problem, **kw)
     t_o, log_o = o.run(steps)
