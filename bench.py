@@ -65,8 +65,8 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     (256, 0, 0, "fused_xy"): 694769.6 * 1024 * 2.0 + 1310721.7 * 1024 * 1.0,
     (256, 0, 0, "fused_z_update"): 1373216.0 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
-    (256, 0, 2, "fused_xy"): 719849.3 * 1024 * 2.0 + 655360.6 * 1024 * 1.0,
-    (256, 0, 2, "fused_z_update"): 1112416.2 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
+    (256, 0, 2, "fused_xy"): 719870.0 * 1024 * 2.0 + 655632.2 * 1024 * 1.0,
+    (256, 0, 2, "fused_z_update"): 1112784.5 * 1024 * 2.0 + 1310976.0 * 1024 * 1.0,
 }
 
 # VALU lane-instructions per launch (SQ_INSTS_VALU x 64), measured with rocprofv3 --pmc on
@@ -74,8 +74,8 @@ MEASURED_TRAFFIC = {
 MEASURED_VALU = {
     (256, 0, 0, "fused_xy"): 327.3e6 * 64,
     (256, 0, 0, "fused_z_update"): 209.7e6 * 64,
-    (256, 0, 2, "fused_xy"): 295.6e6 * 64,
-    (256, 0, 2, "fused_z_update"): 209.6e6 * 64,
+    (256, 0, 2, "fused_xy"): 286.7e6 * 64,
+    (256, 0, 2, "fused_z_update"): 201.4e6 * 64,
 }
 # every SIMD issuing one wave64 VALU instruction per 4 clocks: 256 CUs x 4 SIMDs x 64 lanes
 # / 4 clocks x 2.4 GHz
