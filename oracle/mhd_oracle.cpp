@@ -448,10 +448,17 @@ void orc_generate_problem_shear(orc_ctx* c, const orc_ic_params* ic) {
   orc_prim_to_cons_all(c);
 }
 
+// Parallel regions only for boxes of at least 32768 interior cells: on the small boxes of the
+// convergence and closed-form checks the fork/join cost outweighs the work, and a thread team
+// that spins beside other processes slows the whole step loop by two orders of magnitude.
+static inline bool orc_par(const orc_ctx* c) {
+  return int64_t(c->ie - c->is + 1) * (c->je - c->js + 1) * (c->ke - c->ks + 1) >= 32768;
+}
+
 // ControlTimestep, KH/mhd.cpp:1511-1561 (single rank: MPIminfind is the identity)
 double orc_control_timestep(orc_ctx* c) {
   double dtmin = 1.0e90;
-#pragma omp parallel for collapse(2) reduction(min : dtmin)
+#pragma omp parallel for collapse(2) reduction(min : dtmin) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) {
@@ -469,7 +476,7 @@ double orc_control_timestep(orc_ctx* c) {
 // EvaluateCh, KH/mhd.cpp:1563-1611
 double orc_evaluate_ch(orc_ctx* c) {
   double chg = 0.0;
-#pragma omp parallel for collapse(2) reduction(max : chg)
+#pragma omp parallel for collapse(2) reduction(max : chg) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) {
@@ -598,7 +605,7 @@ void orc_numerical_flux(orc_ctx* c, int dir, double chg) {
   const int d = dir - 1;
   const int eflux = c->cfg.energy_flux;
   const int iu = c->ie + (d == 0), ju = c->je + (d == 1), ku = c->ke + (d == 2);
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (orc_par(c))
   for (int k = c->ks; k <= ku; ++k)
     for (int j = c->js; j <= ju; ++j)
       for (int i = c->is; i <= iu; ++i) {
@@ -616,7 +623,7 @@ void orc_numerical_flux(orc_ctx* c, int dir, double chg) {
 // (x2a(j+1)-x2a(j), x3a(k+1)-x3a(k)); the reference indexes them with i, which is the
 // same spacing up to round-off on the uniform cubic grids it runs.
 void orc_update_conserv(orc_ctx* c, double dt) {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) {
@@ -634,7 +641,7 @@ void orc_update_conserv(orc_ctx* c, double dt) {
 // DampPsi, KH/mhd.cpp:1613-1633
 void orc_damp_psi(orc_ctx* c, double dt, double chg) {
   const double alphabp = c->cfg.alphabp;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) {
@@ -648,7 +655,7 @@ void orc_damp_psi(orc_ctx* c, double dt, double chg) {
 // UpdatePrimitvP, KH/mhd.cpp:1447-1509
 void orc_update_primitive(orc_ctx* c) {
   const double gam = c->cfg.gam;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) {
@@ -678,7 +685,7 @@ void orc_update_primitive(orc_ctx* c) {
 void orc_save_cons(orc_ctx* c) { c->U0 = c->U; }
 
 void orc_average_saved(orc_ctx* c) {
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (orc_par(c))
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i)
@@ -724,7 +731,7 @@ double orc_run(orc_ctx* c, int nsteps, double t0, double* dt_log) {
 void orc_diagnostics(orc_ctx* c, double* out) {
   const int nk = c->ke - c->ks + 1;
   std::vector<double> vol(nk), mix(nk), vy2(nk), dmax(nk);
-#pragma omp parallel for
+#pragma omp parallel for if (orc_par(c))
   for (int kk = 0; kk < nk; ++kk) {
     const int k = c->ks + kk;
     double sv = 0.0, sm = 0.0, sy = 0.0, md = 0.0;
