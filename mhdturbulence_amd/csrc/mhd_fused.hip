@@ -1230,6 +1230,14 @@ bool fused_configure(FusedArgs& a, bool alone) {
   if (a.mixed) {
     const int lanes64 = (a.g.nx + 63) / 64 * 64;
     if (4 * (lanes64 - a.g.nx) <= lanes64) best = 64;
+  } else {
+    // FP64: 64-cell strips whenever at most an eighth of their lanes idle.  Their y chunks
+    // may be twice as long (46 rows against 23) and they take the DPP shifts of full-width
+    // strips: at 408^3, 600^3 and 798^3 (9, 6 and 4 % of the 64-cell lanes idle) the xy
+    // kernel ran 12-17 % faster and the step 7-10 %; at 150^3 (22 % idle) 32-cell strips
+    // stay faster
+    const int lanes64 = (a.g.nx + 63) / 64 * 64;
+    if (8 * (lanes64 - a.g.nx) <= lanes64) best = 64;
   }
   a.sw = best;
   a.ns = (a.g.nx + best - 1) / best;
